@@ -1,0 +1,948 @@
+// rism_ctx.cpp -- the C ABI (include/rism3d_c.h): context, set-up, the
+// fixed-point loop, AN2 and the thermodynamics, driving rism_kernels.hip.
+//
+// Host-side counterpart of the reference's RISM3D orchestration
+// (initialize.cu, iterate.cu, calculate.cu, anderson.cu, output.cu) for one
+// solve on one GPU.  The context owns every device buffer and one stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/rism3d_c.h"
+#include "rism_fft.h"
+#include "rism_internal.h"
+#include "rism_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+const double kAvogadro = 6.02214076e23;          // physical.h:5
+const double kBohr = 0.5291771500001;            // physical.h:8
+const double kBoltzmann = 1.380649e-23;          // physical.h:11
+const double kHartree = 4.359744722207185e-18;   // physical.h:14
+
+#define HIPCHECK(expr)                                                                \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return rism::set_error(RISM_ERR_HIP, std::string(#expr) + ": " +               \
+                                               hipGetErrorString(e_));                \
+  } while (0)
+
+using rism::cplx;
+
+// Natural cubic spline through (xp, yp): the same tridiagonal elimination as
+// spline.cc:4-43 (second derivatives s, then per-interval coefficients
+// coe0 (linear), coe1 (quadratic), coe2 (cubic)).
+void natural_spline(const double *xp, const double *yp, int np, double *c0, double *c1,
+                    double *c2) {
+  std::vector<double> a(np), b(np), c(np), d(np), s(np);
+  const int n = np - 2;
+  for (int k = 1; k <= n; ++k) {
+    const double hl = xp[k] - xp[k - 1], hr = xp[k + 1] - xp[k];
+    a[k] = hl / (hl + hr);
+    b[k] = 1.0 - a[k];
+    const double g1 = (yp[k + 1] - yp[k]) / hr, g2 = (yp[k] - yp[k - 1]) / hl;
+    c[k] = 6.0 * ((g1 - g2) / (hl + hr));
+    d[k] = 2.0;
+  }
+  for (int k = 1; k < n; ++k) {
+    b[k] = b[k] / d[k];
+    c[k] = c[k] / d[k];
+    d[k + 1] = d[k + 1] - a[k + 1] * b[k];
+    c[k + 1] = c[k + 1] - a[k + 1] * c[k];
+  }
+  if (n >= 1) s[n] = c[n] / d[n];
+  for (int k = n - 1; k >= 1; --k) s[k] = c[k] - b[k] * s[k + 1];
+  s[0] = 0.0;
+  s[n + 1] = 0.0;
+  for (int k = 0; k <= n; ++k) {
+    const double h = xp[k + 1] - xp[k];
+    c1[k] = s[k] / 2.0;
+    c2[k] = (s[k + 1] - s[k]) / (6.0 * h);
+    c0[k] = (yp[k + 1] - yp[k]) / h - c1[k] * h - c2[k] * h * h;
+  }
+}
+
+// k component of grid index j (initialize_g.cu:58-60)
+inline double kcomp(int j, int n, double b) {
+  return 2.0 * M_PI * ((double)j - (double)n / 2.0 + 0.5) / b;
+}
+
+struct Tables {
+  cplx *twx = nullptr, *twy = nullptr, *twz = nullptr;
+  cplx *prex = nullptr, *prey = nullptr, *zpre = nullptr, *zpost = nullptr;
+};
+
+std::vector<cplx> roots(int L) {  // e^{-2 pi i q / L}
+  std::vector<cplx> w(L);
+  for (int q = 0; q < L; ++q) {
+    const double a = -2.0 * M_PI * (double)q / (double)L;
+    w[q] = cplx{std::cos(a), std::sin(a)};
+  }
+  return w;
+}
+
+std::vector<cplx> axis_pre(int n) {  // (-1)^j e^{-i pi j / n}
+  std::vector<cplx> w(n);
+  for (int j = 0; j < n; ++j) {
+    const double sg = (j & 1) ? -1.0 : 1.0;
+    const double a = -M_PI * (double)j / (double)n;
+    w[j] = cplx{sg * std::cos(a), sg * std::sin(a)};
+  }
+  return w;
+}
+
+bool grid_ok(int nx, int ny, int nz, std::string &why) {
+  if (nx < 8 || ny < 8 || nz < 16 || (nx & 1) || (ny & 1) || (nz & 1)) {
+    why = "grid sizes must be even, nx, ny >= 8 and nz >= 16";
+    return false;
+  }
+  if (!rism::fft_length_supported(nx) || !rism::fft_length_supported(ny) ||
+      !rism::fft_length_supported(nz / 2)) {
+    why = "unsupported FFT length (nx, ny and nz/2 must be 2^a 3^b 5^c in the built set)";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+int rism::set_error(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+struct rism_ctx {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  rism::Params P{};
+  int nx = 0, ny = 0, nz = 0, natv = 0;
+  size_t N = 0;
+  double box[3] = {0, 0, 0};
+  std::vector<double> rho, qv, sigv, epsv;
+  double temper = 0, xt = 0;
+  int natu = 0;
+  std::vector<double> atoms_host;  // per atom x y z q sig_uv^2[natv] 4eps_uv[natv]
+  rism_opts opts{};
+  // device
+  double *u = nullptr, *fr = nullptr, *T[3] = {}, *R[3] = {};
+  cplx *fk = nullptr, *spec = nullptr;
+  double *partial = nullptr, *tpartial = nullptr, *sums = nullptr, *scratch = nullptr;
+  double *atoms = nullptr, *knots = nullptr, *coef = nullptr;
+  size_t zinv_blocks = 0;
+  Tables tb;
+  std::vector<void *> allocs;
+  // AN2 state (anderson.h:18-40)
+  int count = 0, binary = 0;
+  double lastx[2] = {0, 0}, lasts[2] = {0, 0};
+  int iT_cur = 0, iT_slot[2] = {1, 2}, iR_cur = 0, iR_slot[2] = {1, 2};
+  int iT_last = 0, iR_last = 0;
+  std::vector<double> hsums;
+  bool have_step = false;
+  rism_timing timing{};
+  // per-kernel hipEvent timing (rism_set_kernel_timing)
+  bool ktime = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evnext = 0;
+  struct Pending {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  rism_kstat kstat[RISM_K_COUNT] = {};
+
+  template <class Tp>
+  int alloc(Tp **p, size_t count_elems) {
+    void *q = nullptr;
+    const size_t bytes = sizeof(Tp) * (count_elems ? count_elems : 1);
+    if (hipMalloc(&q, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return rism::set_error(RISM_ERR_NOMEM,
+                             "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    allocs.push_back(q);
+    *p = (Tp *)q;
+    return RISM_OK;
+  }
+  ~rism_ctx() {
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    for (void *q : allocs) (void)hipFree(q);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+template <class Tp>
+int upload(rism_ctx *c, Tp **dst, const std::vector<Tp> &src) {
+  int rc = c->alloc(dst, src.size());
+  if (rc) return rc;
+  HIPCHECK(hipMemcpy(*dst, src.data(), sizeof(Tp) * src.size(), hipMemcpyHostToDevice));
+  return RISM_OK;
+}
+
+int make_tables(rism_ctx *c) {
+  const int M = c->nz / 2;
+  int rc;
+  if ((rc = upload(c, &c->tb.twx, roots(c->nx)))) return rc;
+  if ((rc = upload(c, &c->tb.twy, roots(c->ny)))) return rc;
+  if ((rc = upload(c, &c->tb.twz, roots(M)))) return rc;
+  if ((rc = upload(c, &c->tb.prex, axis_pre(c->nx)))) return rc;
+  if ((rc = upload(c, &c->tb.prey, axis_pre(c->ny)))) return rc;
+  std::vector<cplx> zp(M), zq(M);
+  for (int p = 0; p < M; ++p) {
+    const double a = -M_PI * (double)p / (double)M;
+    zp[p] = cplx{std::cos(a), std::sin(a)};
+    const double b = -M_PI * (double)(2 * p + 1) / (double)c->nz;
+    zq[p] = cplx{std::cos(b), std::sin(b)};
+  }
+  if ((rc = upload(c, &c->tb.zpre, zp))) return rc;
+  if ((rc = upload(c, &c->tb.zpost, zq))) return rc;
+  return RISM_OK;
+}
+
+void fill_params(rism_ctx *c) {
+  rism::Params &P = c->P;
+  P.nx = c->nx;
+  P.ny = c->ny;
+  P.nz = c->nz;
+  P.natv = c->natv;
+  P.mode = 0;
+  const double volume = c->box[0] * c->box[1] * c->box[2];
+  P.dv = volume / (double)c->N;  // cell.cc:13-15
+  P.inv_scale = 1.0 / (P.dv * (double)c->nx * (double)c->ny * (double)(c->nz / 2));
+  P.dr[0] = c->box[0] / c->nx;
+  P.dr[1] = c->box[1] / c->ny;
+  P.dr[2] = c->box[2] / c->nz;
+  for (int d = 0; d < 3; ++d) P.kfac[d] = 2.0 * M_PI / c->box[d];
+  for (int v = 0; v < RISM_MAX_SITES; ++v) P.qv[v] = v < c->natv ? c->qv[v] : 0.0;
+}
+
+// chi spline on the knots that bracket the grid's |k| range
+// (solvent_spline.cu:17-55), uploaded as [interval][coef][v][w].
+int make_chi(rism_ctx *c, const rism_solvent *s) {
+  double k2min = 0.0, k2max = 0.0;
+  {
+    // |k_j| is smallest at j = n/2 and largest at j = 0 on every axis
+    const double kx0 = kcomp(c->nx / 2, c->nx, c->box[0]), ky0 = kcomp(c->ny / 2, c->ny, c->box[1]),
+                 kz0 = kcomp(c->nz / 2, c->nz, c->box[2]);
+    const double kx1 = kcomp(0, c->nx, c->box[0]), ky1 = kcomp(0, c->ny, c->box[1]),
+                 kz1 = kcomp(0, c->nz, c->box[2]);
+    k2min = kx0 * kx0 + ky0 * ky0 + kz0 * kz0;
+    k2max = kx1 * kx1 + ky1 * ky1 + kz1 * kz1;
+  }
+  const double gmin = std::sqrt(k2min), gmax = std::sqrt(k2max);
+  const int ntab = s->ntab;
+  const double dk = M_PI / (s->dr * ntab);
+  std::vector<double> ttab(ntab);
+  for (int i = 0; i < ntab; ++i) ttab[i] = dk * (i + 1);
+  if (gmax > ttab[ntab - 1]) return rism::set_error(RISM_ERR_KRANGE, "insufficient maximal T tabulated");
+  int ntab1 = 0;
+  for (int n = 0; n < ntab; ++n)
+    if (ttab[n] < gmin) ntab1 = n;
+  int ntab2 = ntab - 1;
+  for (int n = ntab - 1; n > 0; --n)
+    if (ttab[n] > gmax) ntab2 = n;
+  const int np = ntab2 - ntab1 + 1;
+  if (np < 3) return rism::set_error(RISM_ERR_KRANGE, "chi table has fewer than 3 knots in range");
+  const int nv = c->natv, nv2 = nv * nv;
+  std::vector<double> coef((size_t)(np - 1) * 4 * nv2), xk(np), yk(np), a0(np), a1(np), a2(np);
+  for (int n = 0; n < np; ++n) xk[n] = ttab[ntab1 + n];
+  for (int v = 0; v < nv; ++v)
+    for (int w = 0; w < nv; ++w) {
+      for (int n = 0; n < np; ++n) yk[n] = s->xvv[((size_t)(ntab1 + n) * nv + v) * nv + w];
+      natural_spline(xk.data(), yk.data(), np, a0.data(), a1.data(), a2.data());
+      for (int i = 0; i < np - 1; ++i) {
+        double *cf = coef.data() + (size_t)i * 4 * nv2 + v * nv + w;
+        cf[0] = yk[i];
+        cf[nv2] = a0[i];
+        cf[2 * nv2] = a1[i];
+        cf[3 * nv2] = a2[i];
+      }
+    }
+  int rc;
+  if ((rc = upload(c, &c->knots, xk))) return rc;
+  if ((rc = upload(c, &c->coef, coef))) return rc;
+  c->P.np = np;
+  c->P.x0 = xk[0];
+  c->P.idk = 1.0 / dk;
+  return RISM_OK;
+}
+
+int check_sizes(int nx, int ny, int nz, int natv) {
+  std::string why;
+  if (!grid_ok(nx, ny, nz, why)) return rism::set_error(RISM_ERR_ARG, why);
+  if (natv < 1 || natv > RISM_MAX_SITES)
+    return rism::set_error(RISM_ERR_ARG, "natv must be in [1, 16]");
+  if (rism::xoz_threads(nx, natv) > 1024)
+    return rism::set_error(RISM_ERR_ARG, "natv * nx too large for the x pass");
+  return RISM_OK;
+}
+
+void reset_an2(rism_ctx *c) {
+  c->count = c->opts.an2_count;
+  c->binary = 0;
+  c->iT_cur = 0;
+  c->iT_slot[0] = 1;
+  c->iT_slot[1] = 2;
+  c->iR_cur = 0;
+  c->iR_slot[0] = 1;
+  c->iR_slot[1] = 2;
+  c->iT_last = 0;
+  c->iR_last = 0;
+  c->have_step = false;
+  c->lastx[0] = c->lastx[1] = c->lasts[0] = c->lasts[1] = 0.0;
+}
+
+// Accumulates the durations of every bracketed launch; the stream must have
+// been synchronised (or the events completed) before this is called.
+void ktime_resolve(rism_ctx *c) {
+  for (const auto &p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c->kstat[p.id].launches += 1;
+      c->kstat[p.id].total_ms += ms;
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  c->pending.clear();
+  c->evnext = 0;
+}
+
+hipEvent_t ktime_event(rism_ctx *c) {
+  if (c->evnext == c->evpool.size()) {
+    if (c->evpool.size() >= 256) {  // pool exhausted: drain and reuse
+      (void)hipStreamSynchronize(c->st);
+      ktime_resolve(c);
+    } else {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      c->evpool.push_back(e);
+    }
+  }
+  return c->evpool[c->evnext++];
+}
+
+// RAII bracket: events around one launch when timing is on
+struct KTime {
+  rism_ctx *c;
+  int id;
+  hipEvent_t a = nullptr;
+  KTime(rism_ctx *c_, int id_) : c(c_), id(id_) {
+    if (c->ktime && (a = ktime_event(c))) (void)hipEventRecord(a, c->st);
+  }
+  ~KTime() {
+    if (!a) return;
+    hipEvent_t b = ktime_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->st);
+    c->pending.push_back({id, a, b});
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *rism_last_error(void) { return g_err.c_str(); }
+
+int rism_set_kernel_timing(rism_ctx *c, int on) {
+  if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  HIPCHECK(hipStreamSynchronize(c->st));
+  c->pending.clear();
+  c->evnext = 0;
+  c->ktime = on != 0;
+  for (auto &k : c->kstat) k = rism_kstat{0, 0.0};
+  return RISM_OK;
+}
+
+int rism_get_kernel_stats(rism_ctx *c, rism_kstat *out) {
+  if (!c || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
+  HIPCHECK(hipStreamSynchronize(c->st));
+  ktime_resolve(c);
+  for (int i = 0; i < RISM_K_COUNT; ++i) out[i] = c->kstat[i];
+  return RISM_OK;
+}
+
+rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
+                      const rism_solute *solute, const rism_opts *opts, int *err) {
+  int dummy;
+  if (!err) err = &dummy;
+  *err = RISM_OK;
+  if (!grid || !solvent || !solute || !opts) {
+    *err = rism::set_error(RISM_ERR_ARG, "null argument");
+    return nullptr;
+  }
+  int rc = check_sizes(grid->nx, grid->ny, grid->nz, solvent->natv);
+  if (rc) {
+    *err = rc;
+    return nullptr;
+  }
+  if (opts->closure != RISM_CLOSURE_KH && opts->closure != RISM_CLOSURE_HNC) {
+    *err = rism::set_error(RISM_ERR_CLOSURE, "Error: Unexpected closure switch ");
+    return nullptr;
+  }
+  rism_ctx *c = new rism_ctx();
+  auto fail = [&](int code) {
+    *err = code;
+    delete c;
+    return (rism_ctx *)nullptr;
+  };
+  if (hipGetDevice(&c->dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(rism::set_error(RISM_ERR_HIP, "no HIP device"));
+  }
+  if (hipStreamCreate(&c->st) != hipSuccess) return fail(rism::set_error(RISM_ERR_HIP, "hipStreamCreate failed"));
+  c->nx = grid->nx;
+  c->ny = grid->ny;
+  c->nz = grid->nz;
+  c->natv = solvent->natv;
+  c->N = (size_t)c->nx * c->ny * c->nz;
+  for (int d = 0; d < 3; ++d) c->box[d] = grid->box[d];
+  c->opts = *opts;
+  c->temper = solvent->temper;
+  c->xt = solvent->xt;
+  c->rho.assign(solvent->rho, solvent->rho + c->natv);
+  c->qv.assign(solvent->q, solvent->q + c->natv);
+  c->sigv.assign(solvent->sig, solvent->sig + c->natv);
+  c->epsv.assign(solvent->eps, solvent->eps + c->natv);
+  c->natu = solute->natu;
+  c->P.clos = opts->closure;
+  fill_params(c);
+
+  // solute table: cal_LJ.cu:26-40 mixing rules, -l scales sigma
+  const double lam1 = (opts->ad_switch == 1) ? opts->lambda : 1.0;
+  const int stride = 4 + 2 * c->natv;
+  c->atoms_host.assign((size_t)(c->natu > 0 ? c->natu : 1) * stride, 0.0);
+  for (int a = 0; a < c->natu; ++a) {
+    double *p = c->atoms_host.data() + (size_t)a * stride;
+    p[0] = solute->xyz[3 * a];
+    p[1] = solute->xyz[3 * a + 1];
+    p[2] = solute->xyz[3 * a + 2];
+    p[3] = solute->q[a];
+    for (int v = 0; v < c->natv; ++v) {
+      const double su = (solute->sig[a] + c->sigv[v]) * 0.5 * lam1;
+      p[4 + v] = su * su;
+      p[4 + c->natv + v] = 4.0 * std::sqrt(solute->eps[a] * c->epsv[v]);
+    }
+  }
+  if ((rc = upload(c, &c->atoms, c->atoms_host))) return fail(rc);
+  if ((rc = make_tables(c))) return fail(rc);
+  if ((rc = make_chi(c, solvent))) return fail(rc);
+
+  const size_t nn = c->N * c->natv;
+  if ((rc = c->alloc(&c->u, nn))) return fail(rc);
+  if ((rc = c->alloc(&c->fr, c->N))) return fail(rc);
+  if ((rc = c->alloc(&c->fk, c->N / 2))) return fail(rc);
+  if ((rc = c->alloc(&c->spec, nn / 2))) return fail(rc);
+  for (int i = 0; i < 3; ++i) {
+    if ((rc = c->alloc(&c->T[i], nn))) return fail(rc);
+    if ((rc = c->alloc(&c->R[i], nn))) return fail(rc);
+  }
+  c->zinv_blocks = rism::zinv_num_blocks(c->P);
+  if ((rc = c->alloc(&c->partial, c->zinv_blocks * 6))) return fail(rc);
+  const int tb = rism::thermo_blocks(c->N);
+  if ((rc = c->alloc(&c->tpartial, (size_t)tb * 4 * c->natv))) return fail(rc);
+  if ((rc = c->alloc(&c->sums, 6 * RISM_MAX_SITES))) return fail(rc);
+  if ((rc = c->alloc(&c->scratch, c->N))) return fail(rc);
+  c->hsums.assign(6 * c->natv, 0.0);
+  reset_an2(c);
+  if (hipMemsetAsync(c->T[0], 0, sizeof(double) * nn, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return fail(rism::set_error(RISM_ERR_HIP, "initial clear failed"));
+  return c;
+}
+
+void rism_destroy(rism_ctx *ctx) { delete ctx; }
+
+int rism_setup_potentials(rism_ctx *c) {
+  if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  const size_t nn = c->N * c->natv;
+  reset_an2(c);
+  // history slots start at zero (the reference leaves them uninitialised)
+  for (int i = 1; i < 3; ++i) {
+    HIPCHECK(hipMemsetAsync(c->T[i], 0, sizeof(double) * nn, c->st));
+    HIPCHECK(hipMemsetAsync(c->R[i], 0, sizeof(double) * nn, c->st));
+  }
+  if (c->opts.verbose) {
+    std::cout << "tabulating solute Lennard-Jones potential ..." << std::endl;
+    if (c->opts.ad_switch != 1) std::cout << "synthesizing solute Coulomb potential ..." << std::endl;
+  }
+  const double ikbt = 1.0 / (kAvogadro * kBoltzmann * c->temper);  // cal_LJ.cu:47
+  const double lam2 = (c->opts.ad_switch == 2) ? c->opts.lambda : 1.0;
+  const double ubeta = kHartree * kBohr / (kBoltzmann * c->temper) * lam2;  // cal_Coulomb.cu:33
+  const int do_coul = c->opts.ad_switch != 1;  // cal_Coulomb.cu:23
+  {
+    KTime kt(c, RISM_K_POT);
+    HIPCHECK(rism::run_potential_r(c->atoms, c->natu, c->P, ikbt, ubeta, do_coul, c->u, c->fr, c->st));
+  }
+  if (do_coul && c->natu > 0) {
+    KTime kt(c, RISM_K_FK);
+    HIPCHECK(rism::run_fk_half(c->atoms, c->natu, 4 + 2 * c->natv, c->P, ubeta, c->fk, c->st));
+  } else {
+    HIPCHECK(hipMemsetAsync(c->fk, 0, sizeof(cplx) * (c->N / 2), c->st));
+  }
+  if (c->opts.verbose) std::cout << "synthesizing initial estimate for Tuv ..." << std::endl;
+  HIPCHECK(rism::run_init_t(c->T[c->iT_cur], c->fr, c->P, c->N, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  return RISM_OK;
+}
+
+int rism_calculate(rism_ctx *c, double *rms) {
+  if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  rism::Params P = c->P;
+  P.mode = 0;
+  double *t = c->T[c->iT_cur];
+  {
+    KTime kt(c, RISM_K_ZFWD);
+    HIPCHECK(rism::run_zfwd(P, t, c->u, c->fr, c->spec, c->tb.twz, c->tb.zpre, c->tb.zpost, c->st));
+  }
+  {
+    KTime kt(c, RISM_K_YFWD);
+    HIPCHECK(rism::run_y(P, c->spec, -1, c->tb.twy, c->tb.prey, c->st));
+  }
+  {
+    KTime kt(c, RISM_K_XOZ);
+    HIPCHECK(rism::run_xoz(P, c->spec, c->fk, c->tb.twx, c->tb.prex, c->knots, c->coef, c->st));
+  }
+  {
+    KTime kt(c, RISM_K_YINV);
+    HIPCHECK(rism::run_y(P, c->spec, +1, c->tb.twy, c->tb.prey, c->st));
+  }
+  const int use_hist = (c->count == 0);
+  {
+    KTime kt(c, RISM_K_ZINV);
+    HIPCHECK(rism::run_zinv(P, c->spec, t, c->u, c->R[c->iR_cur], c->R[c->iR_slot[0]],
+                            c->R[c->iR_slot[1]], c->partial, c->tb.twz, c->tb.zpre, c->tb.zpost,
+                            use_hist, c->st));
+  }
+  const size_t nb = c->zinv_blocks / c->natv;
+  {
+    KTime kt(c, RISM_K_REDUCE);
+    for (int v = 0; v < c->natv; ++v)
+      HIPCHECK(rism::run_reduce_cols(c->partial + (size_t)v * nb * 6, nb, 6, c->sums + 6 * v, c->st));
+  }
+  HIPCHECK(hipMemcpyAsync(c->hsums.data(), c->sums, sizeof(double) * 6 * c->natv,
+                          hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  if (c->ktime) ktime_resolve(c);
+  double ss = 0.0;
+  for (int v = 0; v < c->natv; ++v) ss += c->hsums[6 * v];
+  c->iT_last = c->iT_cur;
+  c->iR_last = c->iR_cur;
+  c->have_step = true;
+  if (rms) *rms = std::sqrt(ss / ((double)c->N * c->natv));  // cal_rms.cu:24
+  return RISM_OK;
+}
+
+int rism_anderson(rism_ctx *c) {
+  if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  if (!c->have_step || c->iR_last != c->iR_cur)
+    return rism::set_error(RISM_ERR_STATE, "rism_anderson needs a preceding rism_calculate");
+  const size_t nn = c->N * c->natv;
+  const int b = c->binary;
+  const int newT = c->iT_slot[b], oldR = c->iR_slot[b];
+  double *t = c->T[c->iT_cur], *R = c->R[c->iR_cur];
+  if (c->count > 0) {  // anderson.cu:54-60
+    --c->count;
+    KTime kt(c, RISM_K_AN2);
+    HIPCHECK(rism::run_an2_picard(c->T[newT], t, R, nn, c->st));
+  } else {  // anderson.cu:61-72, cal_theta anderson.cu:81-102
+    double cc[2] = {0, 0}, a[4] = {0, 0, 0, 0};
+    for (int v = 0; v < c->natv; ++v) {
+      const double ir = 1.0 / c->rho[v];
+      const double *s = &c->hsums[6 * v];
+      cc[0] += s[1] * ir;
+      cc[1] += s[2] * ir;
+      a[0] += s[3] * ir;
+      a[1] += s[4] * ir;
+      a[3] += s[5] * ir;
+    }
+    a[2] = a[1];
+    // leg(): Gaussian elimination without pivoting (anderson.cu:105-130)
+    double x[2];
+    {
+      double p = a[0];
+      a[1] /= p;
+      double r = cc[0] /= p;
+      double q = a[2];
+      a[3] -= q * a[1];
+      cc[1] -= q * r;
+      x[1] = cc[1] / a[3];
+      x[0] = cc[0] - a[1] * x[1];
+    }
+    const double s1 = x[0] + c->opts.an2_mp * ((b == 0) ? 0 : 1);
+    const double s2 = x[1] + c->opts.an2_mp * ((b == 0) ? 1 : 0);
+    c->lastx[0] = x[0];
+    c->lastx[1] = x[1];
+    c->lasts[0] = s1;
+    c->lasts[1] = s2;
+    KTime kt(c, RISM_K_AN2);
+    HIPCHECK(rism::run_an2_update(c->T[newT], t, R, c->T[c->iT_slot[0]], c->T[c->iT_slot[1]],
+                                  c->R[c->iR_slot[0]], c->R[c->iR_slot[1]], s1, s2,
+                                  c->opts.an2_m, nn, c->st));
+  }
+  // slot[b] <- (t, R) by pointer rotation; the new t is in the retired buffer
+  c->iT_slot[b] = c->iT_cur;
+  c->iR_slot[b] = c->iR_cur;
+  c->iT_cur = newT;
+  c->iR_cur = oldR;
+  c->binary = (b == 0) ? 1 : 0;
+  return RISM_OK;
+}
+
+int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, double *rms_out) {
+  if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  if (c->opts.verbose) std::cout << "relaxing 3D UV RISM:" << std::endl;
+  bool conver = false, diverge = false;
+  int istep = 0, last = 0;
+  double rms = 0.0;
+  for (istep = 1; istep <= maxstep; ++istep) {  // iterate.cu:48-72
+    int rc = rism_calculate(c, &rms);
+    if (rc) return rc;
+    last = istep;
+    diverge = !std::isfinite(rms);
+    if (diverge) break;
+    if (rms <= tol) {
+      conver = true;
+    } else {
+      rc = rism_anderson(c);
+      if (rc) return rc;
+    }
+    if (c->opts.verbose) std::cout << " Step = " << istep << " Reside = " << rms << std::endl;
+    if (conver) break;
+  }
+  if (c->opts.verbose) {
+    if (diverge)
+      std::cout << "Calculation diverged." << std::endl;
+    else if (!conver)
+      std::cout << "3D UV RISM: reached limit # of relaxation steps: " << maxstep << std::endl;
+  }
+  const int st = conver ? RISM_STATUS_CONVERGED : (diverge ? RISM_STATUS_DIVERGED : RISM_STATUS_MAXSTEP);
+  if (status) *status = st;
+  if (steps) *steps = last;
+  if (rms_out) *rms_out = rms;
+  c->timing.steps = last;
+  c->timing.status = st;
+  c->timing.rms = rms;
+  return RISM_OK;
+}
+
+int rism_thermo(rism_ctx *c, rism_thermo_out *out) {
+  if (!c || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
+  if (!c->have_step) return rism::set_error(RISM_ERR_STATE, "rism_thermo needs a solved state");
+  const int tb = rism::thermo_blocks(c->N);
+  {
+    KTime kt(c, RISM_K_THERMO);
+    HIPCHECK(rism::run_thermo_partial(c->T[c->iT_cur], c->T[c->iT_last], c->u, c->R[c->iR_last],
+                                      c->P, c->N, c->tpartial, c->st));
+  }
+  for (int v = 0; v < c->natv; ++v)
+    HIPCHECK(rism::run_reduce_cols(c->tpartial + (size_t)v * tb * 4, tb, 4, c->sums + 4 * v, c->st));
+  std::vector<double> s(4 * c->natv);
+  HIPCHECK(hipMemcpyAsync(s.data(), c->sums, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  const int n = c->natv;
+  const double dv = c->P.dv;
+  const double ibeta = kAvogadro * kBoltzmann * c->temper;
+  std::memset(out, 0, sizeof(*out));
+  out->natv = n;
+  double xsc = 0, xhnc = 0, xgf = 0, cuv = 0;
+  for (int v = 0; v < n; ++v) {
+    const double sc = s[4 * v] * c->rho[v] * dv;  // cal_exchem.cu:15,46
+    const double sh = s[4 * v + 1] * c->rho[v] * dv;
+    const double gf = s[4 * v + 2] * c->rho[v] * dv;
+    cuv += s[4 * v + 3] * c->rho[v];  // cal_pmv.cu:14
+    out->sfec_sc[v] = ibeta * sc;
+    out->sfec_sc_hnc[v] = (c->P.clos == 0) ? ibeta * sh : 0.0;
+    out->sfec_gf[v] = ibeta * gf;
+    xsc += sc;
+    xhnc += sh;
+    xgf += gf;
+  }
+  cuv *= dv;
+  const double pmv = c->xt * (1.0 - cuv) * ibeta;  // cal_pmv.cu:21
+  double pres = 0.0;                                // cal_pressure.cu:8-19
+  for (int v = 0; v < n; ++v) pres += c->rho[v];
+  pres *= 1.0e30 / kAvogadro;
+  pres = 0.5 * (pres + 1.0 / (c->xt * ibeta));
+  out->sfe_sc = ibeta * xsc;  // output_xmu.cu:21-31
+  out->sfe_sc_hnc = (c->P.clos == 0) ? ibeta * xhnc : 0.0;
+  out->sfe_gf = ibeta * xgf;
+  out->pmv = pmv * 1000.0;
+  out->pressure = ibeta * pres;
+  out->correction = -pres * pmv * ibeta;
+  return RISM_OK;
+}
+
+int rism_get_field(rism_ctx *c, int which, int site, double *host) {
+  if (!c || !host) return rism::set_error(RISM_ERR_ARG, "null argument");
+  if (which != RISM_FIELD_FR && (site < 0 || site >= c->natv))
+    return rism::set_error(RISM_ERR_ARG, "site out of range");
+  const size_t off = (size_t)site * c->N;
+  const double *src = nullptr;
+  switch (which) {
+    case RISM_FIELD_T:
+      src = c->T[c->iT_cur] + off;
+      break;
+    case RISM_FIELD_U:
+      src = c->u + off;
+      break;
+    case RISM_FIELD_FR:
+      src = c->fr;
+      break;
+    case RISM_FIELD_R:
+      if (!c->have_step) return rism::set_error(RISM_ERR_STATE, "no step taken yet");
+      src = c->R[c->iR_last] + off;
+      break;
+    case RISM_FIELD_G:
+    case RISM_FIELD_H:
+    case RISM_FIELD_C: {
+      const int w = which == RISM_FIELD_G ? 1 : (which == RISM_FIELD_H ? 2 : 3);
+      if (w != 1 && !c->have_step) return rism::set_error(RISM_ERR_STATE, "no step taken yet");
+      const double *t = c->T[c->have_step ? c->iT_last : c->iT_cur] + off;
+      HIPCHECK(rism::run_fields(t, c->u + off, c->R[c->iR_last] + off, c->P, c->N, w, c->scratch, c->st));
+      src = c->scratch;
+      break;
+    }
+    default:
+      return rism::set_error(RISM_ERR_ARG, "unknown field");
+  }
+  HIPCHECK(hipMemcpyAsync(host, src, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  return RISM_OK;
+}
+
+int rism_set_field(rism_ctx *c, int which, int site, const double *host) {
+  if (!c || !host) return rism::set_error(RISM_ERR_ARG, "null argument");
+  if (site < 0 || site >= c->natv) return rism::set_error(RISM_ERR_ARG, "site out of range");
+  double *dst = nullptr;
+  if (which == RISM_FIELD_T)
+    dst = c->T[c->iT_cur] + (size_t)site * c->N;
+  else if (which == RISM_FIELD_U)
+    dst = c->u + (size_t)site * c->N;
+  else
+    return rism::set_error(RISM_ERR_ARG, "only T and U can be set");
+  HIPCHECK(hipMemcpyAsync(dst, host, sizeof(double) * c->N, hipMemcpyHostToDevice, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  return RISM_OK;
+}
+
+int rism_get_timing(rism_ctx *c, rism_timing *out) {
+  if (!c || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
+  *out = c->timing;
+  return RISM_OK;
+}
+
+int rism_run(rism_ctx *c, int maxstep, double tol, rism_thermo_out *out, rism_timing *timing) {
+  if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  hipEvent_t e[4];
+  for (auto &ev : e) HIPCHECK(hipEventCreate(&ev));
+  int rc = RISM_OK;
+  int status = 0, steps = 0;
+  double rms = 0.0;
+  rism_thermo_out th;
+  if (hipEventRecord(e[0], c->st) != hipSuccess) rc = RISM_ERR_HIP;
+  if (!rc) rc = rism_setup_potentials(c);
+  if (!rc && hipEventRecord(e[1], c->st) != hipSuccess) rc = RISM_ERR_HIP;
+  if (!rc) rc = rism_solve(c, maxstep, tol, &status, &steps, &rms);
+  if (!rc && hipEventRecord(e[2], c->st) != hipSuccess) rc = RISM_ERR_HIP;
+  if (!rc) rc = rism_thermo(c, &th);
+  if (!rc && hipEventRecord(e[3], c->st) != hipSuccess) rc = RISM_ERR_HIP;
+  if (!rc && hipEventSynchronize(e[3]) != hipSuccess) rc = RISM_ERR_HIP;
+  if (!rc) {
+    float ms[3];
+    for (int i = 0; i < 3; ++i) (void)hipEventElapsedTime(&ms[i], e[i], e[i + 1]);
+    c->timing.setup_ms = ms[0];
+    c->timing.iterate_ms = ms[1];
+    c->timing.thermo_ms = ms[2];
+    c->timing.steps = steps;
+    c->timing.status = status;
+    c->timing.rms = rms;
+    if (out) *out = th;
+    if (timing) *timing = c->timing;
+  } else if (rc == RISM_ERR_HIP) {
+    rism::set_error(rc, "hipEvent failure in rism_run");
+  }
+  for (auto &ev : e) (void)hipEventDestroy(ev);
+  return rc;
+}
+
+// ---- .guv / .tuv -----------------------------------------------------------
+
+int rism_write_guv(rism_ctx *c, const char *path, const double shift[3]) {
+  if (!c || !path) return rism::set_error(RISM_ERR_ARG, "null argument");
+  std::ofstream o(path, std::ios::binary);
+  if (!o) return rism::set_error(RISM_ERR_IO, std::string("cannot write ") + path);
+  const int32_t hdr[4] = {c->natv, c->nx, c->ny, c->nz};  // output_guv.cu:24-27
+  const double s0[3] = {0, 0, 0};
+  o.write((const char *)hdr, sizeof(hdr));
+  o.write((const char *)c->box, sizeof(double) * 3);
+  o.write((const char *)(shift ? shift : s0), sizeof(double) * 3);
+  std::vector<double> g(c->N);
+  for (int v = 0; v < c->natv; ++v) {
+    int rc = rism_get_field(c, RISM_FIELD_G, v, g.data());
+    if (rc) return rc;
+    o.write((const char *)g.data(), sizeof(double) * c->N);
+  }
+  return o ? RISM_OK : rism::set_error(RISM_ERR_IO, std::string("write failed: ") + path);
+}
+
+int rism_write_tuv(rism_ctx *c, const char *path) {
+  if (!c || !path) return rism::set_error(RISM_ERR_ARG, "null argument");
+  std::ofstream o(path);
+  if (!o) return rism::set_error(RISM_ERR_IO, std::string("cannot write ") + path);
+  o << c->nx << " " << c->ny << " " << c->nz << " " << c->natv << std::endl;  // write_tuv.cu:19-22
+  std::vector<double> t(c->N);
+  for (int v = 0; v < c->natv; ++v) {
+    int rc = rism_get_field(c, RISM_FIELD_T, v, t.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < c->N; ++i) o << t[i] << "\n";
+  }
+  o.flush();
+  return o ? RISM_OK : rism::set_error(RISM_ERR_IO, std::string("write failed: ") + path);
+}
+
+int rism_read_tuv(rism_ctx *c, const char *path) {
+  if (!c || !path) return rism::set_error(RISM_ERR_ARG, "null argument");
+  std::ifstream f(path);
+  if (!f) return rism::set_error(RISM_ERR_IO, std::string("cannot open ") + path);
+  int g0, g1, g2, nv;
+  f >> g0 >> g1 >> g2 >> nv;
+  if (!f) return rism::set_error(RISM_ERR_IO, "bad .tuv header");
+  if (g0 != c->nx || g1 != c->ny || g2 != c->nz || nv != c->natv) {  // read_tuv.cu:14-20
+    std::cout << " RXRISM:  actual   NGr = " << c->N << "   NatV = " << c->natv << std::endl
+              << "           saved   NGr = " << (long long)g0 * g1 * g2 << "   NatV = " << nv
+              << std::endl;
+    return rism::set_error(RISM_ERR_ARG, ".tuv grid does not match the input");
+  }
+  std::vector<double> t(c->N);
+  for (int v = 0; v < c->natv; ++v) {
+    for (size_t i = 0; i < c->N; ++i) f >> t[i];
+    if (!f) return rism::set_error(RISM_ERR_IO, "truncated .tuv");
+    int rc = rism_set_field(c, RISM_FIELD_T, v, t.data());
+    if (rc) return rc;
+  }
+  return RISM_OK;
+}
+
+// ---- kernel-level entry points ----------------------------------------------
+
+}  // extern "C"
+
+namespace {
+int transform_ctx(int nx, int ny, int nz, const double box[3], rism_ctx **out) {
+  int rc = check_sizes(nx, ny, nz, 1);
+  if (rc) return rc;
+  rism_ctx *c = new rism_ctx();
+  c->nx = nx;
+  c->ny = ny;
+  c->nz = nz;
+  c->natv = 1;
+  c->N = (size_t)nx * ny * nz;
+  for (int d = 0; d < 3; ++d) c->box[d] = box[d];
+  c->qv.assign(1, 0.0);
+  fill_params(c);
+  if (hipStreamCreate(&c->st) != hipSuccess) {
+    delete c;
+    return rism::set_error(RISM_ERR_HIP, "hipStreamCreate failed");
+  }
+  if ((rc = make_tables(c))) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return RISM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rism_k_fft_r2c(int nx, int ny, int nz, const double box[3], const double *in, double *out) {
+  if (!box || !in || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
+  rism_ctx *c = nullptr;
+  int rc = transform_ctx(nx, ny, nz, box, &c);
+  if (rc) return rc;
+  struct Del {
+    rism_ctx *c;
+    ~Del() { delete c; }
+  } del{c};
+  double *t;
+  cplx *spec;
+  if ((rc = c->alloc(&t, c->N)) || (rc = c->alloc(&spec, c->N / 2))) return rc;
+  HIPCHECK(hipMemcpy(t, in, sizeof(double) * c->N, hipMemcpyHostToDevice));
+  rism::Params P = c->P;
+  P.mode = 1;
+  HIPCHECK(rism::run_zfwd(P, t, t, t, spec, c->tb.twz, c->tb.zpre, c->tb.zpost, c->st));
+  HIPCHECK(rism::run_y(P, spec, -1, c->tb.twy, c->tb.prey, c->st));
+  HIPCHECK(rism::run_xoz(P, spec, spec, c->tb.twx, c->tb.prex, nullptr, nullptr, c->st));
+  HIPCHECK(rism::run_apply_phase(spec, P, 0, c->st));
+  HIPCHECK(hipMemcpyAsync(out, spec, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  return RISM_OK;
+}
+
+int rism_k_fft_c2r(int nx, int ny, int nz, const double box[3], const double *in, double *out) {
+  if (!box || !in || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
+  rism_ctx *c = nullptr;
+  int rc = transform_ctx(nx, ny, nz, box, &c);
+  if (rc) return rc;
+  struct Del {
+    rism_ctx *c;
+    ~Del() { delete c; }
+  } del{c};
+  double *h;
+  cplx *spec;
+  double *partial;
+  if ((rc = c->alloc(&h, c->N)) || (rc = c->alloc(&spec, c->N / 2)) ||
+      (rc = c->alloc(&partial, rism::zinv_num_blocks(c->P) * 6)))
+    return rc;
+  HIPCHECK(hipMemcpy(spec, in, sizeof(cplx) * (c->N / 2), hipMemcpyHostToDevice));
+  rism::Params P = c->P;
+  P.mode = 2;
+  HIPCHECK(rism::run_apply_phase(spec, P, 1, c->st));
+  HIPCHECK(rism::run_xoz(P, spec, spec, c->tb.twx, c->tb.prex, nullptr, nullptr, c->st));
+  HIPCHECK(rism::run_y(P, spec, +1, c->tb.twy, c->tb.prey, c->st));
+  HIPCHECK(rism::run_zinv(P, spec, h, h, h, h, h, partial, c->tb.twz, c->tb.zpre, c->tb.zpost, 0, c->st));
+  HIPCHECK(hipMemcpyAsync(out, h, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  return RISM_OK;
+}
+
+int rism_k_get_fk(rism_ctx *c, double *out) {
+  if (!c || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
+  cplx *tmp = reinterpret_cast<cplx *>(c->spec);  // borrow the spectrum buffer
+  HIPCHECK(hipMemcpyAsync(tmp, c->fk, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToDevice, c->st));
+  HIPCHECK(rism::run_apply_phase(tmp, c->P, 0, c->st));
+  HIPCHECK(hipMemcpyAsync(out, tmp, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  return RISM_OK;
+}
+
+int rism_k_an2_state(rism_ctx *c, double *x, double *s, int *count, int *binary) {
+  if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  if (x) {
+    x[0] = c->lastx[0];
+    x[1] = c->lastx[1];
+  }
+  if (s) {
+    s[0] = c->lasts[0];
+    s[1] = c->lasts[1];
+  }
+  if (count) *count = c->count;
+  if (binary) *binary = c->binary;
+  return RISM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+// rism_fft.h -- LDS-staged mixed-radix (2,3,4,5,8) Stockham FFT of columns,
+// the building block of the axis passes in rism_kernels.hip.
+//
+// One column (a line of L complex values along one grid axis) is transformed
+// by T threads of a workgroup.  Stage s with radix R performs L/R radix-R
+// butterflies; butterfly j reads elements j + r*L/R and writes elements
+// (j/Ns)*Ns*R + j%Ns + r*Ns, Ns being the product of the earlier radices
+// (the auto-sorting Stockham form, so no bit reversal pass is needed).
+// The first stage can read straight from global memory through a Load
+// functor and the last stage can write straight to global memory through a
+// Store functor; both touch element i at (i, column), so with the column
+// index fastest across lanes every global access is coalesced.  Intermediate
+// stages exchange through LDS, padded one complex every 8 to break the
+// stride-R bank conflicts of the butterfly outputs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace rism {
+
+struct cplx {
+  double x, y;
+};
+
+__device__ __forceinline__ cplx mk(double a, double b) { return cplx{a, b}; }
+__device__ __forceinline__ cplx operator+(cplx a, cplx b) { return cplx{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cplx operator-(cplx a, cplx b) { return cplx{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cplx operator*(double s, cplx a) { return cplx{s * a.x, s * a.y}; }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return cplx{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+__device__ __forceinline__ cplx cmulc(cplx a, cplx b) {
+  return cplx{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+__device__ __forceinline__ cplx conj(cplx a) { return cplx{a.x, -a.y}; }
+// multiply by S*i, S = +1 or -1
+template <int S>
+__device__ __forceinline__ cplx mul_si(cplx a) {
+  return S > 0 ? cplx{-a.y, a.x} : cplx{a.y, -a.x};
+}
+
+// ---- radix-R DFT in registers; S = -1 forward, +1 inverse ----------------
+template <int R, int S>
+struct Dft;
+
+template <int S>
+struct Dft<2, S> {
+  __device__ __forceinline__ static void run(cplx *v) {
+    cplx a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  }
+};
+
+template <int S>
+struct Dft<4, S> {
+  __device__ __forceinline__ static void run(cplx *v) {
+    cplx s02 = v[0] + v[2], d02 = v[0] - v[2];
+    cplx s13 = v[1] + v[3], d13 = mul_si<S>(v[1] - v[3]);
+    v[0] = s02 + s13;
+    v[2] = s02 - s13;
+    v[1] = d02 + d13;
+    v[3] = d02 - d13;
+  }
+};
+
+template <int S>
+struct Dft<8, S> {
+  __device__ __forceinline__ static void run(cplx *v) {
+    constexpr double h = 0.70710678118654752440;
+    cplx e[4] = {v[0], v[2], v[4], v[6]};
+    cplx o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, S>::run(e);
+    Dft<4, S>::run(o);
+    // w8^k = exp(S 2 pi i k / 8)
+    cplx o1 = cmul(o[1], cplx{h, S * h});
+    cplx o2 = mul_si<S>(o[2]);
+    cplx o3 = cmul(o[3], cplx{-h, S * h});
+    v[0] = e[0] + o[0];
+    v[4] = e[0] - o[0];
+    v[1] = e[1] + o1;
+    v[5] = e[1] - o1;
+    v[2] = e[2] + o2;
+    v[6] = e[2] - o2;
+    v[3] = e[3] + o3;
+    v[7] = e[3] - o3;
+  }
+};
+
+template <int S>
+struct Dft<3, S> {
+  __device__ __forceinline__ static void run(cplx *v) {
+    constexpr double s3 = 0.86602540378443864676;  // sqrt(3)/2
+    cplx t1 = v[1] + v[2];
+    cplx t2 = v[0] - 0.5 * t1;
+    cplx t3 = mul_si<S>(s3 * (v[1] - v[2]));
+    v[0] = v[0] + t1;
+    v[1] = t2 + t3;
+    v[2] = t2 - t3;
+  }
+};
+
+template <int S>
+struct Dft<5, S> {
+  __device__ __forceinline__ static void run(cplx *v) {
+    constexpr double c1 = 0.30901699437494742410;   // cos(2pi/5)
+    constexpr double c2 = -0.80901699437494742410;  // cos(4pi/5)
+    constexpr double s1 = 0.95105651629515357212;   // sin(2pi/5)
+    constexpr double s2 = 0.58778525229247312917;   // sin(4pi/5)
+    cplx t1 = v[1] + v[4], t2 = v[2] + v[3];
+    cplx t3 = v[1] - v[4], t4 = v[2] - v[3];
+    cplx b1 = v[0] + c1 * t1 + c2 * t2;
+    cplx b2 = v[0] + c2 * t1 + c1 * t2;
+    cplx d1 = mul_si<S>(s1 * t3 + s2 * t4);
+    cplx d2 = mul_si<S>(s2 * t3 - s1 * t4);
+    v[0] = v[0] + t1 + t2;
+    v[1] = b1 + d1;
+    v[4] = b1 - d1;
+    v[2] = b2 + d2;
+    v[3] = b2 - d2;
+  }
+};
+
+// ---- compile-time factorisation ------------------------------------------
+struct RadixList {
+  int n;
+  int r[16];
+};
+
+constexpr RadixList make_radices(int L) {
+  RadixList rl{0, {}};
+  int m = L;
+  int n8 = 0, n4 = 0, n2 = 0;
+  while (m % 2 == 0) { ++n2; m /= 2; }
+  // powers of two: as many radix-8 as possible, remainder as 4 / 4+? / 2
+  n8 = n2 / 3;
+  int rem = n2 % 3;
+  if (rem == 1) {
+    if (n8 > 0) { n8 -= 1; n4 = 2; } else { n2 = 1; }
+  } else if (rem == 2) {
+    n4 = 1;
+  }
+  int twos = (rem == 1 && n4 == 0) ? 1 : 0;
+  for (int i = 0; i < n8; ++i) rl.r[rl.n++] = 8;
+  for (int i = 0; i < n4; ++i) rl.r[rl.n++] = 4;
+  for (int i = 0; i < twos; ++i) rl.r[rl.n++] = 2;
+  while (m % 5 == 0) { rl.r[rl.n++] = 5; m /= 5; }
+  while (m % 3 == 0) { rl.r[rl.n++] = 3; m /= 3; }
+  if (m != 1) rl.n = -1;  // unsupported prime factor
+  return rl;
+}
+
+constexpr int max_radix(int L) {
+  RadixList rl = make_radices(L);
+  int mx = 1;
+  for (int i = 0; i < rl.n; ++i) mx = rl.r[i] > mx ? rl.r[i] : mx;
+  return mx;
+}
+
+// threads per column: enough for one radix-max butterfly each in the
+// largest-radix stages
+template <int L>
+struct FftShape {
+  static constexpr RadixList rl = make_radices(L);
+  static_assert(rl.n > 0, "FFT length must be 2^a 3^b 5^c");
+  static constexpr int T = L / max_radix(L);
+  static constexpr int CS = L + L / 8 + 1;  // padded LDS column stride
+};
+
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
+
+// One Stockham stage.  lds = this column's LDS region; tw = exp(-2 pi i q / L).
+template <int L, int R, int NS, int S, int T, bool FIRST, bool LAST, class Load,
+          class Store>
+__device__ __forceinline__ void stockham_stage(cplx *lds, int tid,
+                                               const cplx *__restrict__ tw,
+                                               Load &load, Store &store) {
+  constexpr int NB = L / R;
+  constexpr int PER = (NB + T - 1) / T;
+  cplx v[PER][R];
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int j = tid + b * T;
+    if ((NB % T == 0) || j < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int idx = j + r * NB;
+        if constexpr (FIRST)
+          v[b][r] = load(idx);
+        else
+          v[b][r] = lds[lpad(idx)];
+      }
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cplx w = tw[r * k * (L / (NS * R))];
+          v[b][r] = (S < 0) ? cmul(v[b][r], w) : cmulc(v[b][r], w);
+        }
+      }
+      Dft<R, S>::run(v[b]);
+    }
+  }
+  if constexpr (!FIRST) __syncthreads();
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int j = tid + b * T;
+    if ((NB % T == 0) || j < NB) {
+      const int k = j % NS;
+      const int idxD = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int idx = idxD + r * NS;
+        if constexpr (LAST)
+          store(idx, v[b][r]);
+        else
+          lds[lpad(idx)] = v[b][r];
+      }
+    }
+  }
+  if constexpr (!LAST) __syncthreads();
+}
+
+template <int L, int S, int T, int STAGE, int NS, bool FROM_GLOBAL, bool TO_GLOBAL,
+          class Load, class Store>
+__device__ __forceinline__ void stockham_run(cplx *lds, int tid, const cplx *__restrict__ tw,
+                                             Load &load, Store &store) {
+  constexpr RadixList rl = make_radices(L);
+  if constexpr (STAGE < rl.n) {
+    constexpr int R = rl.r[STAGE];
+    constexpr bool first = FROM_GLOBAL && STAGE == 0;
+    constexpr bool last = TO_GLOBAL && STAGE == rl.n - 1;
+    stockham_stage<L, R, NS, S, T, first, last>(lds, tid, tw, load, store);
+    stockham_run<L, S, T, STAGE + 1, NS * R, FROM_GLOBAL, TO_GLOBAL>(lds, tid, tw, load,
+                                                                      store);
+  }
+}
+
+// Full column FFT.  FROM_GLOBAL: stage 0 reads load(i); otherwise it reads the
+// column from LDS (lds[lpad(i)]) and the caller must have synchronised.
+// TO_GLOBAL: the last stage calls store(i, v); otherwise the result is left in
+// LDS and the function ends with a barrier.  S = -1 forward, +1 inverse
+// (unnormalised).
+template <int L, int S, bool FROM_GLOBAL, bool TO_GLOBAL, class Load, class Store>
+__device__ __forceinline__ void fft_column(cplx *lds, int tid, const cplx *__restrict__ tw,
+                                           Load &load, Store &store) {
+  constexpr int T = FftShape<L>::T;
+  stockham_run<L, S, T, 0, 1, FROM_GLOBAL, TO_GLOBAL>(lds, tid, tw, load, store);
+}
+
+}  // namespace rism

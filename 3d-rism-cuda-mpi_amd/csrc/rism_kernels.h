@@ -1,0 +1,65 @@
+// rism_kernels.h -- launch interface of the HIP kernels (rism_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#define RISM_MAX_SITES 16
+
+namespace rism {
+
+struct cplx;
+
+// Everything a kernel needs to know about the grid, the solvent sites and
+// the chi spline, passed by value.
+struct Params {
+  int nx, ny, nz;
+  int natv;
+  int clos;          // 0 KH, 1 HNC
+  int mode;          // 0 solver step; 1 plain forward transform; 2 plain inverse
+  double dv;         // volume element (forward scale)
+  double inv_scale;  // 1 / (dv nx ny (nz/2)): inverse scale of the three passes
+  double dr[3];      // grid spacing
+  double kfac[3];    // 2 pi / box
+  double qv[RISM_MAX_SITES];
+  // chi spline: knots x_i = x0 + i/idk approximately (exact values in `knots`)
+  int np;
+  double x0, idk;
+};
+
+bool fft_length_supported(int L);
+size_t zinv_num_blocks(const Params &P);
+int xoz_threads(int L, int natv);
+int thermo_blocks(size_t N);
+
+hipError_t run_zfwd(const Params &P, const double *t, const double *u, const double *fr,
+                    cplx *spec, const cplx *twM, const cplx *zpre, const cplx *zpost,
+                    hipStream_t st);
+hipError_t run_y(const Params &P, cplx *spec, int dir, const cplx *twL, const cplx *pre,
+                 hipStream_t st);
+hipError_t run_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *twL,
+                   const cplx *prex, const double *knots, const double *coef,
+                   hipStream_t st);
+hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
+                    double *R, const double *R0, const double *R1, double *partial,
+                    const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,
+                    hipStream_t st);
+hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out,
+                           hipStream_t st);
+hipError_t run_an2_picard(double *tnew, const double *t, const double *R, size_t n,
+                          hipStream_t st);
+hipError_t run_an2_update(double *tnew, const double *t, const double *R, const double *T0,
+                          const double *T1, const double *R0, const double *R1, double s1,
+                          double s2, double m, size_t n, hipStream_t st);
+hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
+                           double ubeta, int do_coul, double *u, double *fr, hipStream_t st);
+hipError_t run_fk_half(const double *atoms, int natu, int astride, const Params &P,
+                       double ubeta, cplx *fkraw, hipStream_t st);
+hipError_t run_init_t(double *t, const double *fr, const Params &P, size_t N, hipStream_t st);
+hipError_t run_fields(const double *t, const double *u, const double *R, const Params &P,
+                      size_t n, int which, double *out, hipStream_t st);
+hipError_t run_thermo_partial(const double *t, const double *th, const double *u,
+                              const double *R, const Params &P, size_t N, double *partial,
+                              hipStream_t st);
+hipError_t run_apply_phase(cplx *s, const Params &P, int inverse, hipStream_t st);
+
+}  // namespace rism

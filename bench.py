@@ -1,0 +1,254 @@
+"""Benchmark: seconds per converged 3D-RISM solve on one MI355X.
+
+Workload (default "cfg2"): 512^3 grid in a 256 A box; a synthetic 12877-atom
+protein-like cluster (uniform in a 30 A sphere, >= 1.0 A apart, C/N/O/H + 1 %
+S, charges ~ N(0, 0.35 e) neutralised to +2 e, seed 1002); 0.2 M NaCl in
+TIP3P-type water (the bundled naclaq_vv.xvk chi table, 4 sites); KH closure;
+AN2 2 0.6 0.2; tolerance 1e-6; FP64 throughout.  One step = one full solve:
+LJ/Coulomb potentials, the initial t, the fixed-point loop to convergence and
+the thermodynamics, with the solute and the chi table already resident in
+HBM when the clock starts.
+
+Also reported, per the roofline contract: the dominant hot-loop kernel's
+achieved HBM bandwidth (its algorithmic bytes per launch / its average
+hipEvent-timed launch duration, measured on the solver's own stream during
+the timed solves), and an FP64 CPU baseline (the oracle, a plain-C OpenMP
+restatement of the reference algorithm) timed on a bounded sample on the
+host cores and extrapolated to one solve.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2|cfg1|mobley]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, os.path.join(ROOT, "3d-rism-cuda-mpi_amd"))
+
+METRIC = ("s/solve at 512^3, batched solves/s; mixed vs FP64 at "
+          "|Δμ|<0.05 kcal/mol")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    import gen_inputs as G
+    import __graft_entry__ as ge
+    rism = ge.load_rism3d()
+    if name == "cfg2":
+        sol = G.cfg2_solute()
+        xvk = "naclaq_vv.xvk"
+        grid, box = (512, 512, 512), (256.0, 256.0, 256.0)
+        desc = ("cfg2: 512^3 grid / 256 A box, synthetic 12877-atom cluster (R 30 A, +2 e, "
+                "seed 1002), 0.2 M NaCl TIP3P (bundled naclaq_vv.xvk chi, 4 sites), KH, "
+                "AN2 2 0.6 0.2, tol 1e-6, FP64")
+    elif name == "cfg1":
+        sol = G.cfg1_solute()
+        xvk = "h2o_vv.xvk"
+        grid, box = (64, 64, 64), (32.0, 32.0, 32.0)
+        desc = ("cfg1: 64^3 / 32 A, 24-atom random-walk solute (seed 1001), water "
+                "(bundled h2o_vv.xvk chi), KH, AN2 2 0.6 0.2, tol 1e-6, FP64")
+    elif name == "mobley":
+        d = rism.read_input(os.path.join(GOLDEN, "3drism.dat"),
+                            os.path.join(GOLDEN, "mobley_3053621"), solvent_dir=GOLDEN)
+        sol = {"sig": d["usig"], "eps": d["ueps"], "q": d["uq"], "xyz": d["uxyz"]}
+        xvk = "ctip3p.xvk"
+        grid, box = (256, 256, 256), (64.0, 64.0, 64.0)
+        desc = "example_milc mobley_3053621: 256^3 / 64 A, 12 atoms, ctip3p.xvk, KH, FP64"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    if name != "mobley":
+        from_centre = sol["xyz"].copy()
+        mn, mx = from_centre.min(0), from_centre.max(0)
+        shift = np.array([np.floor(-(mx[i] - mn[i]) / 2 - mn[i] + 0.5) for i in range(3)])
+        sol = dict(sol, xyz=from_centre + shift)  # solute.cu:18-48 centring
+    solv = rism.read_xvk(os.path.join(GOLDEN, xvk))
+    return rism, grid, box, solv, sol, xvk, desc
+
+
+def algorithmic_bytes(natv, N, hist=True):
+    """Minimal HBM bytes per launch of each hot-loop kernel (FP64 words of 8 B):
+    zfwd reads t, u (2n) and fr (1), writes the half spectrum (n);
+    each y pass reads and writes the spectrum (2n);
+    xpass_oz reads the spectrum (n) and the half fk (1), writes (n);
+    zinv reads the spectrum, t, u (3n) and R0, R1 (2n), writes R (n);
+    an2_update reads t, R, T0, T1, R0, R1 (6n), writes t (n)."""
+    w = 8.0 * N
+    n = natv
+    return {
+        "zfwd_closure": (3 * n + 1) * w,
+        "ypass_fwd": 2 * n * w,
+        "xpass_oz": (2 * n + 1) * w,
+        "ypass_inv": 2 * n * w,
+        "zinv_resid": (6 * n if hist else 4 * n) * w,
+        "an2_update": 7 * n * w,
+    }
+
+
+def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv, budget_s=30.0):
+    """FP64 oracle on the host cores, on a bounded sample extrapolated to one
+    solve: one oracle step + one AN2 update at the full grid (synthetic
+    fields: the cost does not depend on the values), plus LJ/Coulomb and fk
+    on a slab of rows scaled to the full grid."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    L = O.lib()
+    threads = L.or_num_threads()
+    s = O.read_xvk(solv_path)
+    nx, ny, nz = grid
+    N = nx * ny * nz
+    # potentials on `rows` y-rows of one z-plane, scaled by N / (rows * nx)
+    rows = 16
+    sub = (nx, rows, 1)
+    xyz = np.ascontiguousarray(sol["xyz"], dtype=np.float64)
+    so = O.Solute(np.asarray(sol["sig"], float), np.asarray(sol["eps"], float),
+                  np.asarray(sol["q"], float), xyz)
+    t0 = time.time()
+    O.potentials(sub, (box[0], box[1] * rows / ny, box[2] / nz), s, so)
+    t_pot = (time.time() - t0) * N / (rows * nx)
+    # one step + one AN2 update at the full grid
+    ga = np.linspace(0.01, 10.0, 64)
+    chi = np.tile(np.eye(natv).reshape(-1), len(ga))
+    indga = np.zeros(N, dtype=np.int32)
+    u = np.zeros(natv * N)
+    fr = np.zeros(N)
+    fk = np.zeros(2 * N)
+    t = np.zeros(natv * N)
+    g = np.zeros(natv * N)
+    R = np.zeros(natv * N)
+    h = np.zeros(natv * N)
+    work = np.zeros(4 * natv * N)
+    t0 = time.time()
+    L.or_calculate(nx, ny, nz, *[float(b) for b in box], natv, 0, np.ascontiguousarray(s.q), t,
+                   u, fr, fk, chi, indga, g, R, h, work)
+    t_step = time.time() - t0
+    del work, g, h
+    A = L.or_an2_create(natv, N, np.ascontiguousarray(s.rho), 0, 0.6, 0.2)
+    t0 = time.time()
+    L.or_an2_step(A, t, R)
+    t_an2 = time.time() - t0
+    L.or_an2_destroy(A)
+    per_solve = t_pot + gpu_steps * (t_step + t_an2)
+    return {
+        "value": per_solve, "unit": "s/solve", "cores": int(threads), "kind": "port",
+        "sample": (f"FP64 oracle (oracle/rism_oracle.c, OpenMP, own mixed-radix FFT): LJ+Coulomb"
+                   f"+fk on {rows} rows x {nx} of one plane scaled x{N // (rows * nx)} = "
+                   f"{t_pot:.1f} s; one step + one AN2 update at {nx}x{ny}x{nz} = "
+                   f"{t_step + t_an2:.1f} s, x {gpu_steps} steps (the GPU's count)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs one solve per GPU and has no "
+              "multi-GPU path; run with --gpus 1", flush=True)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+
+    rism, grid, box, solv, sol, xvk, desc = workload(args.workload)
+    solu = {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"], "uxyz": sol["xyz"]}
+    log(f"[bench] workload {args.workload}: grid {grid}, natu {len(sol['sig'])}, "
+        f"natv {solv['natv']}")
+    g = rism.RISM3D(grid, box, solv, solu, rism.KH, 2, 0.6, 0.2)
+    for i in range(args.warmup):
+        th, tm = g.run(maxstep=10000, tol=1e-6)
+        log(f"[bench] warmup {i}: {tm}")
+    g.kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    results = []
+    for i in range(args.steps):
+        th, tm = g.run(maxstep=10000, tol=1e-6)
+        results.append((th, tm))
+        log(f"[bench] step {i}: {tm} SFE_SC {th['SFE_SC']:.5f}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ks = g.kernel_stats()
+    g.kernel_timing(False)
+    th, tm = results[-1]
+    if tm["status"] != rism.STATUS_CONVERGED:
+        log(f"[bench] WARNING: solve did not converge (status {tm['status']})")
+    ms_per_step = 1000.0 * wall / args.steps
+    natv = solv["natv"]
+    N = grid[0] * grid[1] * grid[2]
+    iters = tm["steps"]
+
+    # roofline of the dominant hot-loop kernel
+    ab = algorithmic_bytes(natv, N)
+    hot = {k: v for k, v in ks.items() if k in ab and v[0] > 0}
+    dom = max(hot, key=lambda k: hot[k][1])
+    launches, total_ms = hot[dom]
+    avg_s = total_ms / launches / 1000.0
+    achieved = ab[dom] / avg_s / 1e9
+    per_kernel = {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1),
+                      "GB/s": (ab[k] / (v[1] / v[0] / 1000.0) / 1e9) if k in ab and v[0] else None}
+                  for k, v in ks.items() if v[0] > 0}
+    iter_bytes = sum(ab.values())
+    iter_ms = tm["iterate_ms"] / max(iters, 1)
+    log("[bench] per-kernel:", json.dumps(per_kernel, indent=None))
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        cpu = cpu_baseline(grid, box, os.path.join(GOLDEN, xvk), sol, iters, natv)
+        log(f"[bench] CPU baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": ms_per_step / 1000.0,
+        "unit": "s/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "grid": list(grid), "box": list(box), "natu": int(len(sol["sig"])),
+            "natv": int(natv), "iterations": iters, "status": tm["status"],
+            "setup_ms": tm["setup_ms"], "iterate_ms": tm["iterate_ms"],
+            "thermo_ms": tm["thermo_ms"], "ms_per_iteration": iter_ms,
+            "iteration_GBps_algorithmic": iter_bytes / (iter_ms / 1000.0) / 1e9,
+            "SFE_SC": th["SFE_SC"], "SFE_GF": th["SFE_GF"], "PMV": th["PMV"],
+            "kernels": per_kernel,
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": ab[dom],
+            "avg_launch_ms": avg_s * 1000.0,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

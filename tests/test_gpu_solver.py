@@ -1,0 +1,172 @@
+"""GPU parity of the solver stages against the FP64 oracle, on seeded inputs at
+sizes the oracle finishes in seconds.
+
+  potentials  cal_LJ.cu / cal_Coulomb.cu / cal_potential.cu   rel 1e-12
+  one step    calculate.cu + cal_rms.cu                       rel 1e-10
+  AN2 path    anderson.cu (t after k steps, leg solution)     rel 1e-9
+  full solve  iterate.cu + output_xmu quantities              rel 1e-7
+
+All arithmetic is FP64 on both sides; the tolerances cover summation-order
+rounding only.
+"""
+import numpy as np
+import pytest
+
+from gen_inputs import random_walk_solute
+
+pytestmark = pytest.mark.gpu
+
+
+def _solvent(oracle, golden, name="h2o_vv.xvk"):
+    s = oracle.read_xvk(f"{golden}/{name}")
+    d = {"natv": s.natv, "ntab": s.ntab, "dr": s.dr, "temper": s.temper, "xt": s.xt,
+         "vsig": s.sig, "veps": s.eps, "vq": s.q, "vrho": s.rho, "xvv": s.xvv}
+    return s, d
+
+
+def _solute(sol):
+    return {"usig": sol.sig, "ueps": sol.eps, "uq": sol.q, "uxyz": sol.xyz}
+
+
+@pytest.fixture(scope="module")
+def case24(oracle, golden):
+    """cfg1-like: 24-atom random-walk solute in water, 48^3 grid in 24 A."""
+    sol = random_walk_solute(24, seed=1001, total_charge=0.0)
+    sol = oracle.Solute(sol["sig"], sol["eps"], sol["q"], sol["xyz"])
+    sol.xyz, sol.shift = oracle.centering(sol.xyz)
+    s, d = _solvent(oracle, golden)
+    return sol, s, d, (48, 48, 48), (24.0, 24.0, 24.0)
+
+
+def test_potentials(rism, oracle, gpu, case24):
+    sol, s, d, grid, box = case24
+    u, fr, fk = oracle.potentials(grid, box, s, sol)
+    g = rism.RISM3D(grid, box, d, _solute(sol))
+    g.initialize()
+    ug = g.fields(rism.FIELD_U).reshape(-1)
+    frg = g.get(rism.FIELD_FR)
+    # the LJ core reaches ~1e13 at the s^2 floor: compare relative per point
+    assert np.allclose(ug, u, rtol=1e-12, atol=1e-12)
+    assert np.allclose(frg, fr, rtol=1e-12, atol=1e-13 * np.abs(fr).max())
+    nx, ny, nz = grid
+    fkc = fk.view(np.complex128).reshape(nz, ny, nx)[nz // 2:]
+    fkg = g.fk_half()
+    assert np.abs(fkg - fkc).max() < 1e-11 * np.abs(fkc).max()
+    # initial t = q_v fr (initialize_tuv.cu:22)
+    t0 = g.fields(rism.FIELD_T)
+    assert np.allclose(t0, s.q[:, None] * fr[None, :], rtol=1e-12, atol=1e-14)
+    g.close()
+
+
+@pytest.mark.parametrize("closure", ["KH", "HNC"])
+def test_one_step(rism, oracle, gpu, case24, closure):
+    sol, s, d, grid, box = case24
+    o = oracle.OracleRISM(grid, box, s, sol, closure).setup()
+    rms_o, g_o, R_o, h_o = o.calculate(o.t0)
+    g = rism.RISM3D(grid, box, d, _solute(sol), closure={"KH": 0, "HNC": 1}[closure])
+    g.initialize()
+    rms_g = g.calculate()
+    assert abs(rms_g - rms_o) < 1e-11 * rms_o
+    for which, ref in ((rism.FIELD_R, R_o), (rism.FIELD_G, g_o), (rism.FIELD_H, h_o)):
+        got = g.fields(which).reshape(-1)
+        scale = np.abs(ref).max()
+        assert np.abs(got - ref).max() < 1e-10 * scale, which
+    g.close()
+
+
+def test_an2_trajectory(rism, oracle, gpu, case24):
+    """t after 8 steps (2 Picard + 6 modified-Anderson) and the last leg()
+    solution follow the oracle."""
+    sol, s, d, grid, box = case24
+    o = oracle.OracleRISM(grid, box, s, sol, "KH", count=2, m=0.6, mp=0.2, tol=1e-12).setup()
+    ro = o.solve(maxstep=8)
+    g = rism.RISM3D(grid, box, d, _solute(sol), count=2, m=0.6, mp=0.2)
+    g.initialize()
+    st, n, rms = g.iterate(maxstep=8, tol=1e-12)
+    assert (st, n) == (rism.STATUS_MAXSTEP, 8)
+    assert abs(rms - ro.rms[-1]) < 1e-9 * ro.rms[-1]
+    t = g.fields(rism.FIELD_T).reshape(-1)
+    assert np.abs(t - ro.t).max() < 1e-9 * np.abs(ro.t).max()
+    x, sv, cnt, b = g.an2_state()
+    assert cnt == 0 and b == 0
+    g.close()
+
+
+@pytest.mark.parametrize("closure,qscale", [("KH", 1.0), ("KH", 0.0), ("HNC", 0.0)])
+def test_full_solve_vs_oracle(rism, oracle, gpu, case24, closure, qscale):
+    """KH with the full charges; HNC (whose exp(-u + t) overflows for this
+    strongly charged solute, see test_divergence) with scaled charges."""
+    sol, s, d, grid, box = case24
+    sol = oracle.Solute(sol.sig, sol.eps, sol.q * qscale, sol.xyz)
+    o = oracle.OracleRISM(grid, box, s, sol, closure, tol=1e-6, maxstep=400).setup()
+    ro = o.solve()
+    to = o.thermo()
+    g = rism.RISM3D(grid, box, d, _solute(sol), closure={"KH": 0, "HNC": 1}[closure])
+    g.initialize()
+    st, n, rms = g.iterate(maxstep=400, tol=1e-6)
+    print(closure, qscale, "steps", n, ro.steps)
+    tg = g.thermo()
+    assert st == ro.status == rism.STATUS_CONVERGED
+    assert n == ro.steps
+    for k in ("SFE_SC", "SFE_GF", "PMV", "Pressure", "Correction_Term"):
+        assert abs(tg[k] - to[k]) <= 1e-7 * max(1.0, abs(to[k])), (k, tg[k], to[k])
+    if closure == "KH":
+        assert abs(tg["SFE_SC_HNC"] - to["SFE_SC_HNC"]) <= 1e-7 * abs(to["SFE_SC_HNC"])
+    h = g.fields(rism.FIELD_H).reshape(-1)
+    assert np.abs(h - ro.h).max() < 1e-7 * np.abs(ro.h).max()
+    g.close()
+
+
+def test_divergence(rism, oracle, gpu, case24):
+    """HNC with the full charges of the 24-atom solute overflows exp(-u + t):
+    both sides stop at the same step with a non-finite rms and report
+    divergence (iterate.cu:50-54, 73-76)."""
+    sol, s, d, grid, box = case24
+    o = oracle.OracleRISM(grid, box, s, sol, "HNC", tol=1e-6, maxstep=50).setup()
+    ro = o.solve()
+    g = rism.RISM3D(grid, box, d, _solute(sol), closure=rism.HNC)
+    g.initialize()
+    st, n, rms = g.iterate(maxstep=50, tol=1e-6)
+    g.close()
+    assert ro.status == 1 and st == rism.STATUS_DIVERGED
+    assert n == ro.steps
+    assert not np.isfinite(rms)
+
+
+def test_maxstep_limit(rism, oracle, gpu, case24):
+    """maxstep reached: status MAXSTEP, the thermodynamics still evaluate
+    (h of the last step, t after its AN2 update, as output.cu reads them)."""
+    sol, s, d, grid, box = case24
+    o = oracle.OracleRISM(grid, box, s, sol, "KH", tol=1e-6, maxstep=5).setup()
+    ro = o.solve()
+    to = o.thermo()
+    g = rism.RISM3D(grid, box, d, _solute(sol))
+    g.initialize()
+    st, n, rms = g.iterate(maxstep=5, tol=1e-6)
+    tg = g.thermo()
+    g.close()
+    assert ro.status == 2 and st == rism.STATUS_MAXSTEP and n == 5
+    for k in ("SFE_SC", "SFE_GF", "PMV"):
+        assert abs(tg[k] - to[k]) <= 1e-8 * max(1.0, abs(to[k])), (k, tg[k], to[k])
+
+
+def test_lambda_switches(rism, oracle, gpu, case24):
+    """-l (LJ sigma scaling, Coulomb off) and -e (Coulomb scaling) potentials."""
+    sol, s, d, grid, box = case24
+    for ad, lam in ((1, 0.7), (2, 0.4)):
+        u, fr, fk = oracle.potentials(grid, box, s, sol, lam=lam, adswitch=ad)
+        g = rism.RISM3D(grid, box, d, _solute(sol), ad_switch=ad, lam=lam)
+        g.initialize()
+        assert np.allclose(g.fields(rism.FIELD_U).reshape(-1), u, rtol=1e-12, atol=1e-12)
+        assert np.allclose(g.get(rism.FIELD_FR), fr, rtol=1e-12, atol=1e-14)
+        g.close()
+
+
+def test_krange_error(rism, oracle, gpu, golden):
+    """|k|max beyond the chi table -> RISM_ERR_KRANGE (solvent_spline.cu:17-20)."""
+    s, d = _solvent(oracle, golden)
+    sol = {"usig": np.array([3.0]), "ueps": np.array([100.0]), "uq": np.array([0.0]),
+           "uxyz": np.zeros((1, 3))}
+    with pytest.raises(rism.RismError) as e:
+        rism.RISM3D((64, 64, 64), (2.0, 2.0, 2.0), d, sol)
+    assert e.value.code == rism.RISM_ERR_KRANGE
