@@ -138,7 +138,7 @@ struct rism_ctx {
   double *u = nullptr, *fr = nullptr, *T[3] = {}, *R[3] = {};
   cplx *fk = nullptr, *spec = nullptr;
   double *partial = nullptr, *tpartial = nullptr, *sums = nullptr, *scratch = nullptr;
-  double *atoms = nullptr, *knots = nullptr, *coef = nullptr;
+  double *atoms = nullptr, *knots = nullptr, *coef = nullptr, *chitab = nullptr;
   size_t zinv_blocks = 0;
   Tables tb;
   std::vector<void *> allocs;
@@ -276,6 +276,18 @@ int make_chi(rism_ctx *c, const rism_solvent *s) {
   c->P.np = np;
   c->P.x0 = xk[0];
   c->P.idk = 1.0 / dk;
+  // cubic box: every |k|^2 is (pi/L)^2 (ox^2 + oy^2 + oz^2) with odd o, i.e.
+  // (pi/L)^2 (8 i + 3); tabulate chi once per shell i
+  c->P.shell = 0;
+  if (!c->opts.chi_spline && c->box[0] == c->box[1] && c->box[1] == c->box[2]) {
+    const long smax = (long)(c->nx - 1) * (c->nx - 1) + (long)(c->ny - 1) * (c->ny - 1) +
+                      (long)(c->nz - 1) * (c->nz - 1);
+    const size_t nshell = (size_t)((smax - 3) / 8 + 1);
+    if ((rc = c->alloc(&c->chitab, nshell * nv2))) return rc;
+    HIPCHECK(rism::run_chi_table(c->knots, c->coef, c->P, M_PI / c->box[0], nshell, c->chitab, c->st));
+    HIPCHECK(hipStreamSynchronize(c->st));
+    c->P.shell = 1;
+  }
   return RISM_OK;
 }
 
@@ -284,8 +296,10 @@ int check_sizes(int nx, int ny, int nz, int natv) {
   if (!grid_ok(nx, ny, nz, why)) return rism::set_error(RISM_ERR_ARG, why);
   if (natv < 1 || natv > RISM_MAX_SITES)
     return rism::set_error(RISM_ERR_ARG, "natv must be in [1, 16]");
-  if (rism::xoz_threads(nx, natv) > 1024)
-    return rism::set_error(RISM_ERR_ARG, "natv * nx too large for the x pass");
+  // the x pass keeps one row of every site in one workgroup, with the
+  // per-site OZ values in registers
+  if (natv > 8 || rism::xoz_threads(nx, natv) > 512)
+    return rism::set_error(RISM_ERR_ARG, "natv must be <= 8 (and <= 4 for nx = 1024)");
   return RISM_OK;
 }
 
@@ -514,7 +528,7 @@ int rism_calculate(rism_ctx *c, double *rms) {
   }
   {
     KTime kt(c, RISM_K_XOZ);
-    HIPCHECK(rism::run_xoz(P, c->spec, c->fk, c->tb.twx, c->tb.prex, c->knots, c->coef, c->st));
+    HIPCHECK(rism::run_xoz(P, c->spec, c->fk, c->tb.twx, c->tb.prex, c->knots, c->coef, c->chitab, c->st));
   }
   {
     KTime kt(c, RISM_K_YINV);
@@ -886,7 +900,7 @@ int rism_k_fft_r2c(int nx, int ny, int nz, const double box[3], const double *in
   P.mode = 1;
   HIPCHECK(rism::run_zfwd(P, t, t, t, spec, c->tb.twz, c->tb.zpre, c->tb.zpost, c->st));
   HIPCHECK(rism::run_y(P, spec, -1, c->tb.twy, c->tb.prey, c->st));
-  HIPCHECK(rism::run_xoz(P, spec, spec, c->tb.twx, c->tb.prex, nullptr, nullptr, c->st));
+  HIPCHECK(rism::run_xoz(P, spec, spec, c->tb.twx, c->tb.prex, nullptr, nullptr, nullptr, c->st));
   HIPCHECK(rism::run_apply_phase(spec, P, 0, c->st));
   HIPCHECK(hipMemcpyAsync(out, spec, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToHost, c->st));
   HIPCHECK(hipStreamSynchronize(c->st));
@@ -912,7 +926,7 @@ int rism_k_fft_c2r(int nx, int ny, int nz, const double box[3], const double *in
   rism::Params P = c->P;
   P.mode = 2;
   HIPCHECK(rism::run_apply_phase(spec, P, 1, c->st));
-  HIPCHECK(rism::run_xoz(P, spec, spec, c->tb.twx, c->tb.prex, nullptr, nullptr, c->st));
+  HIPCHECK(rism::run_xoz(P, spec, spec, c->tb.twx, c->tb.prex, nullptr, nullptr, nullptr, c->st));
   HIPCHECK(rism::run_y(P, spec, +1, c->tb.twy, c->tb.prey, c->st));
   HIPCHECK(rism::run_zinv(P, spec, h, h, h, h, h, partial, c->tb.twz, c->tb.zpre, c->tb.zpost, 0, c->st));
   HIPCHECK(hipMemcpyAsync(out, h, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->st));

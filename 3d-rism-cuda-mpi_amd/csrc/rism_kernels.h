@@ -16,6 +16,7 @@ struct Params {
   int natv;
   int clos;          // 0 KH, 1 HNC
   int mode;          // 0 solver step; 1 plain forward transform; 2 plain inverse
+  int shell;         // 1: chi from the cubic-box shell table, 0: spline at |k|
   double dv;         // volume element (forward scale)
   double inv_scale;  // 1 / (dv nx ny (nz/2)): inverse scale of the three passes
   double dr[3];      // grid spacing
@@ -38,7 +39,9 @@ hipError_t run_y(const Params &P, cplx *spec, int dir, const cplx *twL, const cp
                  hipStream_t st);
 hipError_t run_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *twL,
                    const cplx *prex, const double *knots, const double *coef,
-                   hipStream_t st);
+                   const double *chitab, hipStream_t st);
+hipError_t run_chi_table(const double *knots, const double *coef, const Params &P,
+                         double kscale, size_t nshell, double *out, hipStream_t st);
 hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
                     double *R, const double *R0, const double *R1, double *partial,
                     const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,

@@ -1,0 +1,407 @@
+// rism_kmisc.hip -- FP64 reductions, the AN2 update, set-up potentials,
+// k-space Coulomb, initial t, derived fields and thermodynamic partial sums.
+#include "rism_common.h"
+
+namespace rism {
+
+// ---------------------------------------------------------------------------
+// FP64 reductions
+// ---------------------------------------------------------------------------
+// Sums nblk rows of `width` doubles into out[width] (one workgroup per column
+// group; pairwise tree within the workgroup, sequential across strides).
+__global__ void __launch_bounds__(256)
+    k_reduce_cols(const double *__restrict__ in, size_t nrow, int width,
+                  double *__restrict__ out) {
+  __shared__ double sh[256];
+  const int col = blockIdx.x;
+  double a = 0.0, comp = 0.0;  // Kahan-compensated per-thread accumulation
+  for (size_t r = threadIdx.x; r < nrow; r += blockDim.x) {
+    const double yv = in[r * width + col] - comp;
+    const double tv = a + yv;
+    comp = (tv - a) - yv;
+    a = tv;
+  }
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[col] = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// AN2 (anderson_cuda.cu:1-22)
+// ---------------------------------------------------------------------------
+// Picard warm-up: slot <- (t, R) is a pointer rotation on the host; the new
+// t = t + R lands in the buffer being retired.
+__global__ void k_an2_picard(double *__restrict__ tnew, const double *__restrict__ t,
+                             const double *__restrict__ R, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) tnew[i] = t[i] + R[i];
+}
+
+// tnew aliases T0 or T1 (the slot being retired); each element is read before
+// it is written by the same thread.
+__global__ void k_an2_update(double *tnew, const double *__restrict__ t,
+                             const double *__restrict__ R, const double *T0, const double *T1,
+                             const double *__restrict__ R0, const double *__restrict__ R1,
+                             double s1, double s2, double mcoef, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ti = t[i], ri = R[i];
+  const double t0 = T0[i], t1 = T1[i];
+  const double uu = ti + s1 * (t0 - ti) + s2 * (t1 - ti);
+  const double vv = ti + ri + s1 * (t0 + R0[i] - ti - ri) + s2 * (t1 + R1[i] - ti - ri);
+  tnew[i] = uu + mcoef * (vv - uu);
+}
+
+// ---------------------------------------------------------------------------
+// set-up: potentials (cal_LJ.cu:56-81, cal_Coulomb.cu:40-62, cal_potential.cu:39-43)
+// ---------------------------------------------------------------------------
+// Each thread owns PX consecutive x points of one (y, z) line and a group of
+// G <= 8 solvent sites starting at v0; solute atoms stream through LDS in
+// tiles of ATILE records (x, y, z, q, sigma_uv^2[natv], 4 eps_uv[natv]).
+// Outputs u[v] = beta sum 4 eps (s^-12 - s^-6) + q_v de, and fr (first group).
+// 1/r comes from the v_rsq_f64 seed refined by two Newton steps, so the pair
+// loop has no FP64 division or square root.
+constexpr int PX = 4;
+constexpr int ATILE = 256;
+
+__device__ __forceinline__ double rsq_nr(double x) {
+  double yv = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  yv = yv * fma(-hx * yv, yv, 1.5);
+  yv = yv * fma(-hx * yv, yv, 1.5);
+  return yv;
+}
+
+template <int G>
+__global__ void __launch_bounds__(256)
+    k_potential_r(const double *__restrict__ atoms, int natu, int v0, Params P, double ikbt,
+                  double ubeta, int do_coul, int write_fr, double *__restrict__ u,
+                  double *__restrict__ fr) {
+  extern __shared__ double sa[];
+  const int nv = P.natv;
+  const int stride = 4 + 2 * nv;
+  const int nxq = (P.nx + PX - 1) / PX;
+  const size_t lines = (size_t)P.ny * P.nz;
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = gid < lines * (size_t)nxq;
+  const int xq = active ? (int)(gid % nxq) : 0;
+  const size_t line = active ? gid / nxq : 0;
+  const int y = (int)(line % P.ny), z = (int)(line / P.ny);
+  const double ry = (double)(y - P.ny / 2) * P.dr[1];
+  const double rz = (double)(z - P.nz / 2) * P.dr[2];
+  double rx[PX];
+#pragma unroll
+  for (int p = 0; p < PX; ++p) rx[p] = (double)(xq * PX + p - P.nx / 2) * P.dr[0];
+  double lj[PX][G];
+  double de[PX], ff[PX];
+#pragma unroll
+  for (int p = 0; p < PX; ++p) {
+    de[p] = 0.0;
+    ff[p] = 0.0;
+#pragma unroll
+    for (int w = 0; w < G; ++w) lj[p][w] = 0.0;
+  }
+  constexpr double icut2 = 1.0 / (2.0e-3 * 2.0e-3);  // s^2 >= 4e-6 (cal_LJ.cu:11-12)
+  for (int a0 = 0; a0 < natu; a0 += ATILE) {
+    const int na = min(ATILE, natu - a0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < na * stride; i += blockDim.x)
+      sa[i] = atoms[(size_t)a0 * stride + i];
+    __syncthreads();
+    for (int a = 0; a < na; ++a) {
+      const double *at = sa + a * stride;
+      const double ax = at[0], ay = at[1], az = at[2], aq = at[3];
+      double sg2[G], e4[G];
+#pragma unroll
+      for (int w = 0; w < G; ++w) {
+        sg2[w] = at[4 + v0 + w];
+        e4[w] = at[4 + nv + v0 + w];
+      }
+      const double dy = ry - ay, dz = rz - az;
+      const double dyz = dy * dy + dz * dz;
+#pragma unroll
+      for (int p = 0; p < PX; ++p) {
+        const double dx = rx[p] - ax;
+        const double r2 = dx * dx + dyz;
+        const double ir = rsq_nr(r2);  // NaN at r2 = 0: both uses below handle it
+        const double ir2 = ir * ir;
+#pragma unroll
+        for (int w = 0; w < G; ++w) {
+          const double s2 = fmin(sg2[w] * ir2, icut2);  // fmin drops the NaN of r = 0
+          const double s6 = s2 * s2 * s2;
+          lj[p][w] += e4[w] * s6 * (s6 - 1.0);
+        }
+        if (do_coul) {
+          const double ra = r2 * ir;
+          if (ra >= 1.0e-5) {
+            const double qr = aq * ir;
+            de[p] += qr;
+            // 1 - e^{-r} rounds to exactly 1 in FP64 once r > 37.5
+            ff[p] += (ra < 38.0) ? qr * (1.0 - exp(-ra)) : qr;
+          } else {
+            ff[p] += aq;
+          }
+        }
+      }
+    }
+  }
+  if (!active) return;
+  const size_t N = (size_t)P.nx * P.ny * P.nz;
+  const size_t rowbase = line * P.nx;
+#pragma unroll
+  for (int p = 0; p < PX; ++p) {
+    const int x = xq * PX + p;
+    if (x >= P.nx) break;
+    const double e = de[p] * ubeta;
+#pragma unroll
+    for (int w = 0; w < G; ++w)
+      u[(size_t)(v0 + w) * N + rowbase + x] = lj[p][w] * ikbt + P.qv[v0 + w] * e;
+    if (write_fr) fr[rowbase + x] = ff[p] * ubeta;
+  }
+}
+
+// fk on the half spectrum (cal_Coulomb.cu:65-79, 90-95), stored raw:
+// fkraw = conj(phi(k)) fk, phi(k) = -i (-1)^(nx/2 + ny/2 + jx + jy + m).
+// Each thread owns FKP consecutive jx of one (jy, m) row; e^{-i kx x_u} is
+// advanced along the row by the recurrence e^{-i (kx + dkx) x_u} =
+// e^{-i kx x_u} e^{-i dkx x_u}.
+constexpr int FKP = 16;
+
+__global__ void __launch_bounds__(256)
+    k_fk_half(const double *__restrict__ atoms, int natu, int astride, Params P, double ubeta,
+              cplx *__restrict__ fkraw) {
+  __shared__ double sa[ATILE * 4];
+  const int M = P.nz / 2;
+  const int nxq = (P.nx + FKP - 1) / FKP;
+  const size_t rows = (size_t)P.ny * M;
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = gid < rows * (size_t)nxq;
+  const int xq = active ? (int)(gid % nxq) : 0;
+  const size_t row = active ? gid / nxq : 0;
+  const int jy = (int)(row % P.ny), m = (int)(row / P.ny);
+  const int jx0 = xq * FKP;
+  const double kx0 = P.kfac[0] * ((double)jx0 - (double)P.nx / 2.0 + 0.5);
+  const double ky = P.kfac[1] * ((double)jy - (double)P.ny / 2.0 + 0.5);
+  const double kz = P.kfac[2] * ((double)(m + P.nz / 2) - (double)P.nz / 2.0 + 0.5);
+  const double dkx = P.kfac[0];
+  double re[FKP], im[FKP];
+#pragma unroll
+  for (int p = 0; p < FKP; ++p) re[p] = im[p] = 0.0;
+  for (int a0 = 0; a0 < natu; a0 += ATILE) {
+    const int na = min(ATILE, natu - a0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < na; i += blockDim.x) {
+      const double *src = atoms + (size_t)(a0 + i) * astride;
+      sa[4 * i] = src[0];
+      sa[4 * i + 1] = src[1];
+      sa[4 * i + 2] = src[2];
+      sa[4 * i + 3] = src[3];
+    }
+    __syncthreads();
+    for (int a = 0; a < na; ++a) {
+      const double ax = sa[4 * a], ay = sa[4 * a + 1], az = sa[4 * a + 2], aq = sa[4 * a + 3];
+      double sn, cs, dsn, dcs;
+      sincos(kx0 * ax + ky * ay + kz * az, &sn, &cs);
+      sincos(dkx * ax, &dsn, &dcs);
+      // e = cos - i sin; step = dcs - i dsn
+      double er = cs, ei = -sn;
+#pragma unroll
+      for (int p = 0; p < FKP; ++p) {
+        re[p] += aq * er;
+        im[p] += aq * ei;
+        const double nr = er * dcs + ei * dsn;
+        const double ni = ei * dcs - er * dsn;
+        er = nr;
+        ei = ni;
+      }
+    }
+  }
+  if (!active) return;
+  const int sgn0 = ((P.nx / 2 + P.ny / 2 + jy + m) & 1) ? -1 : 1;
+#pragma unroll
+  for (int p = 0; p < FKP; ++p) {
+    const int jx = jx0 + p;
+    if (jx >= P.nx) break;
+    const double kx = P.kfac[0] * ((double)jx - (double)P.nx / 2.0 + 0.5);
+    const double rk2 = kx * kx + ky * ky + kz * kz;
+    const double rk4i = 1.0 / (rk2 * (rk2 + 1.0));
+    const double f = 4.0 * M_PI * rk4i * ubeta;
+    const double fre = f * re[p], fim = f * im[p];
+    // conj(phi) = i * s
+    const double s = ((jx & 1) ? -sgn0 : sgn0);
+    fkraw[row * P.nx + jx] = mk(-s * fim, s * fre);
+  }
+}
+
+// t = q_v fr (initialize_tuv.cu:19-23)
+__global__ void k_init_t(double *__restrict__ t, const double *__restrict__ fr, Params P,
+                         size_t N) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N * P.natv) return;
+  const int v = (int)(i / N);
+  t[i] = P.qv[v] * fr[i - (size_t)v * N];
+}
+
+// g = closure(t, u), h = R + g - 1 (the h of the step that produced R)
+__global__ void k_fields(const double *__restrict__ t, const double *__restrict__ u,
+                         const double *__restrict__ R, Params P, size_t n, int which,
+                         double *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double g = closure_g(P.clos, t[i], u[i]);
+  if (which == 1)
+    out[i] = g;
+  else if (which == 2)
+    out[i] = R[i] + g - 1.0;
+  else
+    out[i] = R[i] + g - 1.0 - t[i];
+}
+
+// ---------------------------------------------------------------------------
+// thermodynamics (cal_exchem.cu:55-155, cal_pmv.cu:27-57): per-block FP64
+// sums of SC (closure-matched), HNC, GF integrands and h - t, per site.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_thermo_partial(const double *__restrict__ t, const double *__restrict__ th,
+                     const double *__restrict__ u, const double *__restrict__ R, Params P,
+                     size_t N, double *__restrict__ partial) {
+  // t: the current t (the integrand's t); th: the t of the step that produced
+  // R, from which h = R + g(th, u) - 1 is rebuilt (equal unless the loop
+  // stopped at maxstep, after an AN2 update)
+  __shared__ double sh[4][256 / 64];
+  const int v = blockIdx.y;
+  const double *tv = t + (size_t)v * N, *uv = u + (size_t)v * N, *Rv = R + (size_t)v * N;
+  const double *thv = th + (size_t)v * N;
+  double s[4] = {0, 0, 0, 0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double tt = tv[i];
+    const double g = closure_g(P.clos, thv[i], uv[i]);
+    const double h = Rv[i] + g - 1.0;
+    const double ehnc = h * 0.5 * tt - (h - tt);
+    const double egf = -(h * 0.5 + 1.0) * (h - tt);
+    const double ekh = (h > 0.0) ? egf : ehnc;
+    s[0] += (P.clos == 0) ? ekh : ehnc;
+    s[1] += ehnc;
+    s[2] += egf;
+    s[3] += h - tt;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double a = s[k];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    s[k] = a;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 4; ++k) sh[k][wv] = s[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double a = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) a += sh[threadIdx.x][w];
+    partial[((size_t)v * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = a;
+  }
+}
+
+// reference-convention spectrum from a raw one (tests): F = phi * D,
+// phi = -i (-1)^(nx/2 + ny/2 + jx + jy + m); scaled by `scale`.
+__global__ void k_apply_phase(cplx *__restrict__ s, Params P, int inverse) {
+  const size_t M = P.nz / 2;
+  const size_t n = M * P.ny * P.nx;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int jx = (int)(i % P.nx), jy = (int)((i / P.nx) % P.ny), m = (int)(i / ((size_t)P.nx * P.ny));
+  const int sg = ((P.nx / 2 + P.ny / 2 + jx + jy + m) & 1) ? -1 : 1;
+  const cplx a = s[i];
+  // phi = -i sg : (x, y) -> sg * (y, -x); conj(phi) = i sg : sg * (-y, x)
+  s[i] = inverse ? mk(-sg * a.y, sg * a.x) : mk(sg * a.y, -sg * a.x);
+}
+
+hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_cols, dim3(width), dim3(256), 0, st, in, nrow, width, out);
+  return hipGetLastError();
+}
+
+
+hipError_t run_an2_picard(double *tnew, const double *t, const double *R, size_t n,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_an2_picard, dim3(nblk(n, 256)), dim3(256), 0, st, tnew, t, R, n);
+  return hipGetLastError();
+}
+
+hipError_t run_an2_update(double *tnew, const double *t, const double *R, const double *T0,
+                          const double *T1, const double *R0, const double *R1, double s1,
+                          double s2, double m, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_an2_update, dim3(nblk(n, 256)), dim3(256), 0, st, tnew, t, R, T0, T1, R0,
+                     R1, s1, s2, m, n);
+  return hipGetLastError();
+}
+
+hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
+                           double ubeta, int do_coul, double *u, double *fr, hipStream_t st) {
+  const size_t nth = (size_t)P.ny * P.nz * ((P.nx + PX - 1) / PX);
+  const size_t lds = (size_t)ATILE * (4 + 2 * P.natv) * sizeof(double);
+  const dim3 grid(nblk(nth, 256)), block(256);
+  for (int v0 = 0; v0 < P.natv; v0 += 8) {
+    const int g = min(8, P.natv - v0);
+    const int wf = v0 == 0;
+    switch (g) {
+#define G_CASE(n)                                                                              \
+  case n:                                                                                      \
+    hipLaunchKernelGGL(k_potential_r<n>, grid, block, lds, st, atoms, natu, v0, P, ikbt, ubeta, \
+                       do_coul, wf, u, fr);                                                    \
+    break;
+      G_CASE(1) G_CASE(2) G_CASE(3) G_CASE(4) G_CASE(5) G_CASE(6) G_CASE(7) G_CASE(8)
+#undef G_CASE
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t run_fk_half(const double *atoms, int natu, int astride, const Params &P,
+                       double ubeta, cplx *fkraw, hipStream_t st) {
+  const size_t nth = (size_t)P.ny * (P.nz / 2) * ((P.nx + FKP - 1) / FKP);
+  hipLaunchKernelGGL(k_fk_half, dim3(nblk(nth, 256)), dim3(256), 0, st, atoms, natu, astride, P,
+                     ubeta, fkraw);
+  return hipGetLastError();
+}
+
+hipError_t run_init_t(double *t, const double *fr, const Params &P, size_t N, hipStream_t st) {
+  hipLaunchKernelGGL(k_init_t, dim3(nblk(N * P.natv, 256)), dim3(256), 0, st, t, fr, P, N);
+  return hipGetLastError();
+}
+
+hipError_t run_fields(const double *t, const double *u, const double *R, const Params &P,
+                      size_t n, int which, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_fields, dim3(nblk(n, 256)), dim3(256), 0, st, t, u, R, P, n, which, out);
+  return hipGetLastError();
+}
+
+int thermo_blocks(size_t N) {
+  size_t b = (N + 255) / 256;
+  return (int)(b < 1024 ? b : 1024);
+}
+
+hipError_t run_thermo_partial(const double *t, const double *th, const double *u,
+                              const double *R, const Params &P, size_t N, double *partial,
+                              hipStream_t st) {
+  dim3 grid(thermo_blocks(N), P.natv);
+  hipLaunchKernelGGL(k_thermo_partial, grid, dim3(256), 0, st, t, th, u, R, P, N, partial);
+  return hipGetLastError();
+}
+
+hipError_t run_apply_phase(cplx *s, const Params &P, int inverse, hipStream_t st) {
+  const size_t n = (size_t)(P.nz / 2) * P.ny * P.nx;
+  hipLaunchKernelGGL(k_apply_phase, dim3(nblk(n, 256)), dim3(256), 0, st, s, P, inverse);
+  return hipGetLastError();
+}
+
+
+}  // namespace rism

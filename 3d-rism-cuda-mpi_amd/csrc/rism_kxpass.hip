@@ -1,0 +1,238 @@
+// rism_kxpass.hip -- pass 3 of one step (x DFT + OZ + inverse x DFT) and
+// the layout conventions shared by all hand-written CDNA4 (gfx950) kernels.
+//
+// One fixed-point step (reference: RISM3D::calculate, calculate.cu:4-65,
+// and RISM3D::cal_rms, cal_rms.cu:14-26) runs as five passes over HBM:
+//
+//   1. zfwd_closure  closure (KH/HNC) + c_sr + real-to-half-complex DFT along
+//                    z, per site   (kh/hnc/trm1mt + 1/3 of FFT3D forward)
+//   2. ypass<-1>     complex DFT along y                 (1/3 of forward)
+//   3. xpass_oz      DFT along x, -q_v fk, site-site OZ contraction with
+//                    chi_vv(|k|), inverse DFT along x    (mqvfk + oz)
+//   4. ypass<+1>     inverse DFT along y
+//   5. zinv_resid    half-complex-to-real inverse DFT along z, closure again,
+//                    residual R = h + 1 - g, FP64 partial sums of R.R and of
+//                    the AN2 Gram entries                (tr + theta30)
+//
+// followed by an elementwise AN2 update (anderson_cuda.cu:1-22).
+//
+// Transform convention (fft3d.cu:81-163, fft3d_cuda.cu:56-128, with the
+// S = +1 phase): F(k) = dv sum_r f(r) e^{-ik.r}, f(r) = V^-1 sum_k F(k) e^{ik.r},
+// r_n = (n - N/2) L/N, k_j = 2 pi (j - N/2 + 1/2)/L.  Because f is real,
+// only the k_z > 0 half is stored.  Spectra in HBM are kept "raw": the
+// unit-modulus per-axis output phases phi(k) of the exact transform are left
+// out (they cancel between the forward and the inverse, and the OZ operator is
+// real), and the stored fk is multiplied by conj(phi) once at set-up.
+//
+// Data layout: real fields [site][z][y][x] (x fastest) in FP64; spectra
+// [site][m][y][x] complex FP64 with m in [0, nz/2) <-> k_z index m + nz/2.
+
+#include "rism_common.h"
+
+namespace rism {
+
+// ---------------------------------------------------------------------------
+// 3. DFT along x, OZ with chi_vv(|k|), inverse DFT along x
+// ---------------------------------------------------------------------------
+// One workgroup per (m, y) row, all sites: natv rows of nx complex in LDS;
+// natv * (nx / max radix) <= 512 threads, natv <= 8.
+// chi_vv is evaluated from the natural cubic spline of the .xvk table at
+// |k| = sqrt(kx^2 + ky^2 + kz^2) (solvent_spline.cu:44-55 evaluates the same
+// spline at the same |k| shell values, splint.cc:4-20 picks the interval).
+// chi_vv(|k|) sources for the OZ step.  ChiTable: the cubic-box shell table
+// (one row of natv^2 values per k^2 shell, built once by k_chi_table);
+// ChiSpline: the spline evaluated at |k| directly (any box).
+struct ChiTable {
+  const double *row;
+  __device__ __forceinline__ double operator()(int pw) const { return row[pw]; }
+};
+struct ChiSpline {
+  const double *cf;
+  double tt;
+  int nv2;
+  __device__ __forceinline__ double operator()(int pw) const {
+    return ((cf[3 * nv2 + pw] * tt + cf[2 * nv2 + pw]) * tt + cf[nv2 + pw]) * tt + cf[pw];
+  }
+};
+
+// OZ at the mirror pair of points p1 (kx) and p2 (-kx) of one row, which
+// share |k| and therefore chi: H_v = sum_w X[v][w] (C_w - q_w fk)
+// (mqvfk + oz, calculate_cuda.cu:32-52).  NVM is a compile-time bound on the
+// site count so the per-site values stay in registers.
+template <int NVM, class Chi>
+__device__ __forceinline__ void oz_pair(cplx *lds, int CS, int p1, int p2, cplx fk1, cplx fk2,
+                                        const Params &P, int nv, const Chi &X) {
+  cplx c1[NVM], c2[NVM];
+#pragma unroll
+  for (int w = 0; w < NVM; ++w) {
+    if (w < nv) {
+      c1[w] = lds[w * CS + p1] - P.qv[w] * fk1;
+      c2[w] = lds[w * CS + p2] - P.qv[w] * fk2;
+    }
+  }
+  // one output site at a time keeps the live chi values to one row of NVM
+#pragma unroll 1
+  for (int v = 0; v < NVM; ++v) {
+    if (v < nv) {
+      cplx a1 = mk(0.0, 0.0), a2 = mk(0.0, 0.0);
+#pragma unroll
+      for (int w = 0; w < NVM; ++w) {
+        if (w < nv) {
+          const double x = X(v * nv + w);
+          a1 = a1 + x * c1[w];
+          a2 = a2 + x * c2[w];
+        }
+      }
+      lds[v * CS + p1] = a1;
+      lds[v * CS + p2] = a2;
+    }
+  }
+}
+
+template <class Chi>
+__device__ __forceinline__ void oz_dispatch(cplx *lds, int CS, int p1, int p2, cplx fk1,
+                                            cplx fk2, const Params &P, const Chi &X) {
+  switch (P.natv) {
+    case 1: oz_pair<1>(lds, CS, p1, p2, fk1, fk2, P, 1, X); break;
+    case 2: oz_pair<2>(lds, CS, p1, p2, fk1, fk2, P, 2, X); break;
+    case 3: oz_pair<3>(lds, CS, p1, p2, fk1, fk2, P, 3, X); break;
+    case 4: oz_pair<4>(lds, CS, p1, p2, fk1, fk2, P, 4, X); break;
+    case 5: oz_pair<5>(lds, CS, p1, p2, fk1, fk2, P, 5, X); break;
+    case 6: oz_pair<6>(lds, CS, p1, p2, fk1, fk2, P, 6, X); break;
+    case 7: oz_pair<7>(lds, CS, p1, p2, fk1, fk2, P, 7, X); break;
+    case 8: oz_pair<8>(lds, CS, p1, p2, fk1, fk2, P, 8, X); break;
+    default: oz_pair<8>(lds, CS, p1, p2, fk1, fk2, P, 8, X); break;
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(512)
+    k_xpass_oz(cplx *__restrict__ spec, const cplx *__restrict__ fkraw, Params P,
+               const cplx *__restrict__ twL, const cplx *__restrict__ prex,
+               const double *__restrict__ knots, const double *__restrict__ coef,
+               const double *__restrict__ chitab) {
+  constexpr int T = FftShape<L>::T, CS = FftShape<L>::CS;
+  extern __shared__ cplx lds[];
+  const int tid = threadIdx.x;
+  const int nv = P.natv;
+  const int v = tid / T, j = tid % T;
+  const int y = blockIdx.x, m = blockIdx.y;
+  const int M = P.nz / 2;
+  const size_t rowstride = (size_t)M * P.ny * L;  // site stride in spec
+  const size_t rowoff = ((size_t)m * P.ny + y) * L;
+  cplx *row = spec + (size_t)v * rowstride + rowoff;
+  cplx *mylds = lds + v * CS;
+  auto load = [&](int i) -> cplx { return cmul(row[i], prex[i]); };
+  auto store = [&](int i, cplx vv) { row[i] = cmulc(vv, prex[i]); };
+  auto nostore = [&](int, cplx) {};
+  fft_column<L, -1, true, false>(mylds, j, twL, load, nostore);
+
+  // OZ at every k of the row, two mirror points (kx, -kx) per iteration
+  const int nthr = nv * T;
+  const int nv2 = nv * nv;
+  const double ky = P.kfac[1] * ((double)y - (double)P.ny / 2.0 + 0.5);
+  const double kz = P.kfac[2] * ((double)(m + P.nz / 2) - (double)P.nz / 2.0 + 0.5);
+  const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates
+  const long syz = oy * oy + oz * oz;
+  for (int jx = tid; jx < L / 2; jx += nthr) {
+    const int jm = L - 1 - jx;
+    const cplx fk1 = fkraw[rowoff + jx], fk2 = fkraw[rowoff + jm];
+    const int p1 = lpad(jx), p2 = lpad(jm);
+    if (P.shell) {
+      const long ox = 2 * jx - L + 1;
+      const long sidx = (ox * ox + syz - 3) >> 3;  // |k|^2 = (pi/L)^2 (8 sidx + 3)
+      oz_dispatch(lds, CS, p1, p2, fk1, fk2, P, ChiTable{chitab + sidx * nv2});
+    } else {
+      const double kx = P.kfac[0] * ((double)jx - (double)L / 2.0 + 0.5);
+      const double kk = sqrt(kx * kx + ky * ky + kz * kz);
+      const int ii = spline_interval(knots, P.np, P.x0, P.idk, kk);
+      oz_dispatch(lds, CS, p1, p2, fk1, fk2, P,
+                  ChiSpline{coef + (size_t)ii * 4 * nv2, kk - knots[ii], nv2});
+    }
+  }
+  __syncthreads();
+  fft_column<L, +1, false, true>(mylds, j, twL, load, store);
+}
+
+// Plain transform along x of the half spectrum (kernel-level entry points):
+// S = -1 forward (pre-twiddle, DFT), S = +1 inverse (DFT^-1, post-twiddle).
+template <int L, int S>
+__global__ void __launch_bounds__(512)
+    k_xpass_plain(cplx *__restrict__ spec, Params P, const cplx *__restrict__ twL,
+                  const cplx *__restrict__ prex) {
+  constexpr int T = FftShape<L>::T, CS = FftShape<L>::CS;
+  extern __shared__ cplx lds[];
+  const int v = threadIdx.x / T, j = threadIdx.x % T;
+  const size_t rowoff = ((size_t)blockIdx.y * P.ny + blockIdx.x) * L;
+  cplx *row = spec + (size_t)v * (P.nz / 2) * P.ny * L + rowoff;
+  auto load = [&](int i) -> cplx { return S < 0 ? cmul(row[i], prex[i]) : row[i]; };
+  auto store = [&](int i, cplx vv) { row[i] = S < 0 ? vv : cmulc(vv, prex[i]); };
+  fft_column<L, S, true, true>(lds + v * CS, j, twL, load, store);
+}
+
+// chi_vv on every k^2 shell of a cubic box: |k| = (pi/L) sqrt(8 i + 3), i.e.
+// the shells of the reference's sorted k^2 list (initialize_g.cu:26-45) with
+// the spline of solvent_spline.cu:44-55 evaluated at each.
+__global__ void k_chi_table(const double *__restrict__ knots, const double *__restrict__ coef,
+                            Params P, double kscale, size_t nshell, double *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshell) return;
+  const double kk = kscale * sqrt((double)(8 * i + 3));
+  const int ii = spline_interval(knots, P.np, P.x0, P.idk, kk);
+  const int nv2 = P.natv * P.natv;
+  const ChiSpline X{coef + (size_t)ii * 4 * nv2, kk - knots[ii], nv2};
+  for (int pw = 0; pw < nv2; ++pw) out[i * nv2 + pw] = X(pw);
+}
+
+template <int L>
+static void launch_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *twL,
+                       const cplx *prex, const double *knots, const double *coef,
+                       const double *chitab, hipStream_t st) {
+  dim3 grid(P.ny, P.nz / 2);
+  const int threads = P.natv * FftShape<L>::T;
+  const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cplx);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xpass_oz<L>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (P.mode == 1)
+    hipLaunchKernelGGL((k_xpass_plain<L, -1>), grid, dim3(threads), lds, st, spec, P, twL, prex);
+  else if (P.mode == 2)
+    hipLaunchKernelGGL((k_xpass_plain<L, 1>), grid, dim3(threads), lds, st, spec, P, twL, prex);
+  else
+    hipLaunchKernelGGL((k_xpass_oz<L>), grid, dim3(threads), lds, st, spec, fkraw, P, twL, prex,
+                       knots, coef, chitab);
+}
+int xoz_threads(int L, int natv) {
+  switch (L) {
+#define X(n) \
+  case n:    \
+    return natv * FftShape<n>::T;
+    RISM_LENGTHS(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+hipError_t run_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *twL,
+                   const cplx *prex, const double *knots, const double *coef,
+                   const double *chitab, hipStream_t st) {
+  switch (P.nx) {
+#define X(n)                                                           \
+  case n:                                                              \
+    launch_xoz<n>(P, spec, fkraw, twL, prex, knots, coef, chitab, st); \
+    break;
+    RISM_LENGTHS(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t run_chi_table(const double *knots, const double *coef, const Params &P,
+                          double kscale, size_t nshell, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_chi_table, dim3((unsigned)((nshell + 255) / 256)), dim3(256), 0, st, knots,
+                     coef, P, kscale, nshell, out);
+  return hipGetLastError();
+}
+
+}  // namespace rism

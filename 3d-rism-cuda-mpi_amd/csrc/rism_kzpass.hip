@@ -1,0 +1,252 @@
+// rism_kzpass.hip -- the z passes of one step: closure + c_sr + real to
+// half-complex DFT along z (pass 1), and half-complex to real inverse DFT
+// along z + residual + FP64 Gram partial sums (pass 5).  Conventions and the
+// five-pass structure are described in rism_kxpass.hip.
+#include "rism_common.h"
+
+namespace rism {
+
+// ---------------------------------------------------------------------------
+// 1. closure + c_sr + real->half-complex DFT along z
+// ---------------------------------------------------------------------------
+// Column = the z-line at (x, y) of one site: nz reals packed as M = nz/2
+// complex z_p = c(2p) + i c(2p+1), pre-twiddled by e^{-i pi p/M}; after the
+// M-point DFT, G(m) = E(m) + w_m O(m) with E, O from Z(m) and Z(M-1-m) and
+// w_m = e^{-2 pi i (m+1/2)/nz}: the z-DFT at the positive half-shifted
+// frequencies.
+template <int M, int B>
+__global__ void __launch_bounds__(B *FftShape<M>::T)
+    k_zfwd_closure(const double *__restrict__ t, const double *__restrict__ u,
+                   const double *__restrict__ fr, Params P, cplx *__restrict__ spec,
+                   const cplx *__restrict__ twM, const cplx *__restrict__ zpre,
+                   const cplx *__restrict__ zpost) {
+  constexpr int T = FftShape<M>::T, CS = FftShape<M>::CS;
+  __shared__ cplx lds[B * CS];
+  const int tid = threadIdx.x;
+  const int c = tid % B, j = tid / B;
+  const int x = blockIdx.x * B + c;
+  const int y = blockIdx.y;
+  const int v = blockIdx.z;
+  const bool ok = x < P.nx;
+  const size_t plane = (size_t)P.nx * P.ny;
+  const size_t N = plane * (size_t)P.nz;
+  const double *tv = t + (size_t)v * N;
+  const double *uv = u + (size_t)v * N;
+  const double q = P.qv[v];
+  const size_t base = (size_t)y * P.nx + (ok ? x : 0);
+  const int clos = P.clos;
+  const double dv = P.dv;
+  cplx *col = lds + c * CS;
+
+  auto cval = [&](int z) -> double {
+    const size_t i = base + (size_t)z * plane;
+    const double tt = tv[i];
+    if (P.mode != 0) return tt * dv;  // plain transform of t
+    const double g = closure_g(clos, tt, uv[i]);
+    return (g - 1.0 - tt + q * fr[i]) * dv;  // trm1mt, times the forward dv
+  };
+  auto load = [&](int p) -> cplx {
+    if (!ok) return mk(0.0, 0.0);
+    return cmul(mk(cval(2 * p), cval(2 * p + 1)), zpre[p]);
+  };
+  auto store = [&](int, cplx) {};
+  fft_column<M, -1, true, false>(col, j, twM, load, store);
+
+  cplx *out = spec + ((size_t)v * M * P.ny + y) * P.nx + x;
+  const size_t mstride = (size_t)P.ny * P.nx;
+  for (int m = j; m < M; m += T) {
+    const cplx a = col[lpad(m)];
+    const cplx b = conj(col[lpad(M - 1 - m)]);
+    const cplx E = 0.5 * (a + b);
+    const cplx O = mul_si<-1>(0.5 * (a - b));
+    if (ok) out[(size_t)m * mstride] = E + cmul(zpost[m], O);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. half-complex->real inverse DFT along z, residual, FP64 partial sums
+// ---------------------------------------------------------------------------
+// Partial sums per workgroup (6 doubles): R.R, R.(R-R0), R.(R-R1),
+// (R-R0)^2, (R-R0)(R-R1), (R-R1)^2 (anderson_cuda.cu:34-40, cal_rms.cu:20).
+template <int M, int B>
+__global__ void __launch_bounds__(B *FftShape<M>::T)
+    k_zinv_resid(const cplx *__restrict__ spec, const double *__restrict__ t,
+                 const double *__restrict__ u, double *__restrict__ R,
+                 const double *__restrict__ R0, const double *__restrict__ R1, Params P,
+                 double *__restrict__ partial, const cplx *__restrict__ twM,
+                 const cplx *__restrict__ zpre, const cplx *__restrict__ zpost, int use_hist) {
+  constexpr int T = FftShape<M>::T, CS = FftShape<M>::CS;
+  constexpr int NT = B * T;
+  __shared__ cplx lds[B * CS];
+  const int tid = threadIdx.x;
+  const int c = tid % B, j = tid / B;
+  const int x = blockIdx.x * B + c;
+  const int y = blockIdx.y;
+  const int v = blockIdx.z;
+  const bool ok = x < P.nx;
+  const size_t plane = (size_t)P.nx * P.ny;
+  const size_t N = plane * (size_t)P.nz;
+  cplx *col = lds + c * CS;
+  const cplx *in = spec + ((size_t)v * M * P.ny + y) * P.nx + (ok ? x : 0);
+  for (int m = j; m < M; m += T) col[lpad(m)] = ok ? in[(size_t)m * plane] : mk(0.0, 0.0);
+  __syncthreads();
+  // G(m) -> Z(m) = E(m) + i O(m), pairs (m, M-1-m) handled by one thread
+  for (int m = j; 2 * m <= M - 1; m += T) {
+    const int mm = M - 1 - m;
+    const cplx a = col[lpad(m)], b = col[lpad(mm)];
+    const cplx Em = 0.5 * (a + conj(b));
+    const cplx Om = cmulc(0.5 * (a - conj(b)), zpost[m]);
+    const cplx Emm = 0.5 * (b + conj(a));
+    const cplx Omm = cmulc(0.5 * (b - conj(a)), zpost[mm]);
+    col[lpad(m)] = Em + mul_si<1>(Om);
+    if (mm != m) col[lpad(mm)] = Emm + mul_si<1>(Omm);
+  }
+  __syncthreads();
+
+  const double *tv = t + (size_t)v * N;
+  const double *uv = u + (size_t)v * N;
+  double *Rv = R + (size_t)v * N;
+  const double *R0v = R0 + (size_t)v * N;
+  const double *R1v = R1 + (size_t)v * N;
+  const size_t base = (size_t)y * P.nx + (ok ? x : 0);
+  const int clos = P.clos;
+  const double scale = P.inv_scale;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  const int plain = P.mode != 0;
+  auto resid = [&](int z, double h) {
+    const size_t i = base + (size_t)z * plane;
+    if (plain) {  // plain inverse transform: write h itself
+      Rv[i] = h;
+      return;
+    }
+    const double g = closure_g(clos, tv[i], uv[i]);
+    const double r = h + 1.0 - g;  // tr (calculate_cuda.cu:58)
+    Rv[i] = r;
+    s[0] += r * r;
+    if (use_hist) {
+      const double d0 = r - R0v[i], d1 = r - R1v[i];
+      s[1] += r * d0;
+      s[2] += r * d1;
+      s[3] += d0 * d0;
+      s[4] += d0 * d1;
+      s[5] += d1 * d1;
+    }
+  };
+  auto load = [&](int) -> cplx { return mk(0.0, 0.0); };
+  auto store = [&](int p, cplx zv) {
+    if (!ok) return;
+    const cplx zz = scale * cmulc(zv, zpre[p]);
+    resid(2 * p, zz.x);
+    resid(2 * p + 1, zz.y);
+  };
+  fft_column<M, +1, false, true>(col, j, twM, load, store);
+
+  // workgroup reduction of the six sums: wave shuffles then LDS when the
+  // workgroup is made of whole waves, LDS only otherwise (tiny grids)
+  __syncthreads();
+  double *red = reinterpret_cast<double *>(lds);
+  int nparts;
+  if constexpr (NT % 64 == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      double a = s[k];
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+      s[k] = a;
+    }
+    const int lane = tid & 63, wv = tid >> 6;
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) red[wv * 6 + k] = s[k];
+    }
+    nparts = NT / 64;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[tid * 6 + k] = s[k];
+    nparts = NT;
+  }
+  __syncthreads();
+  if (tid < 6) {
+    double a = 0.0;
+    for (int w = 0; w < nparts; ++w) a += red[w * 6 + tid];
+    const size_t blk = ((size_t)v * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    partial[blk * 6 + tid] = a;
+  }
+}
+
+template <int M>
+static void launch_zfwd(const Params &P, const double *t, const double *u, const double *fr,
+                        cplx *spec, const cplx *twM, const cplx *zpre, const cplx *zpost,
+                        hipStream_t st) {
+  constexpr int B = tile_cols<M>();
+  dim3 grid((P.nx + B - 1) / B, P.ny, P.natv);
+  hipLaunchKernelGGL((k_zfwd_closure<M, B>), grid, dim3(B * FftShape<M>::T), 0, st, t, u, fr,
+                     P, spec, twM, zpre, zpost);
+}
+template <int M>
+static void launch_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
+                        double *R, const double *R0, const double *R1, double *partial,
+                        const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,
+                        hipStream_t st) {
+  constexpr int B = tile_cols<M>();
+  dim3 grid((P.nx + B - 1) / B, P.ny, P.natv);
+  hipLaunchKernelGGL((k_zinv_resid<M, B>), grid, dim3(B * FftShape<M>::T), 0, st, spec, t, u,
+                     R, R0, R1, P, partial, twM, zpre, zpost, use_hist);
+}
+bool fft_length_supported(int L) {
+  switch (L) {
+#define X(n) case n:
+    RISM_LENGTHS(X)
+#undef X
+    return true;
+    default:
+      return false;
+  }
+}
+size_t zinv_num_blocks(const Params &P) {
+  const int M = P.nz / 2;
+  int B = 16;
+  switch (M) {
+#define X(n) \
+  case n:    \
+    B = tile_cols<n>(); \
+    break;
+    RISM_LENGTHS(X)
+#undef X
+    default:
+      break;
+  }
+  return (size_t)((P.nx + B - 1) / B) * P.ny * P.natv;
+}
+hipError_t run_zfwd(const Params &P, const double *t, const double *u, const double *fr,
+                    cplx *spec, const cplx *twM, const cplx *zpre, const cplx *zpost,
+                    hipStream_t st) {
+  switch (P.nz / 2) {
+#define X(n)                                                 \
+  case n:                                                    \
+    launch_zfwd<n>(P, t, u, fr, spec, twM, zpre, zpost, st); \
+    break;
+    RISM_LENGTHS(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
+                    double *R, const double *R0, const double *R1, double *partial,
+                    const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,
+                    hipStream_t st) {
+  switch (P.nz / 2) {
+#define X(n)                                                                             \
+  case n:                                                                                \
+    launch_zinv<n>(P, spec, t, u, R, R0, R1, partial, twM, zpre, zpost, use_hist, st); \
+    break;
+    RISM_LENGTHS(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace rism

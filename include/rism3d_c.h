@@ -95,6 +95,8 @@ typedef struct {
                          /* 2 = -e (Coulomb scaling)                          */
   double lambda;         /* the -l / -e value                                 */
   int verbose;           /* print the reference's progress lines            */
+  int chi_spline;        /* 1: evaluate chi_vv(|k|) from the spline at every  */
+                         /* k; 0: per-shell table when the box is cubic       */
 } rism_opts;
 
 /* Thermodynamics of the converged state (output.cu:23-46, output_xmu.cu). */

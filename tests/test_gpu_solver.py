@@ -54,7 +54,7 @@ def test_potentials(rism, oracle, gpu, case24):
     assert np.abs(fkg - fkc).max() < 1e-11 * np.abs(fkc).max()
     # initial t = q_v fr (initialize_tuv.cu:22)
     t0 = g.fields(rism.FIELD_T)
-    assert np.allclose(t0, s.q[:, None] * fr[None, :], rtol=1e-12, atol=1e-14)
+    assert np.allclose(t0, s.q[:, None] * fr[None, :], rtol=1e-12, atol=1e-13 * np.abs(fr).max())
     g.close()
 
 
@@ -158,7 +158,8 @@ def test_lambda_switches(rism, oracle, gpu, case24):
         g = rism.RISM3D(grid, box, d, _solute(sol), ad_switch=ad, lam=lam)
         g.initialize()
         assert np.allclose(g.fields(rism.FIELD_U).reshape(-1), u, rtol=1e-12, atol=1e-12)
-        assert np.allclose(g.get(rism.FIELD_FR), fr, rtol=1e-12, atol=1e-14)
+        assert np.allclose(g.get(rism.FIELD_FR), fr, rtol=1e-12,
+                           atol=1e-13 * max(np.abs(fr).max(), 1.0))
         g.close()
 
 
