@@ -40,4 +40,18 @@ constexpr int tile_cols() {
 
 static inline unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Workgroups of a 1D grid are handed to the 8 XCDs of the MI355X round-robin
+// by linear id.  This bijection renumbers them so that each XCD executes one
+// contiguous range of logical ids, in order: neighbouring tiles (which share
+// cache lines and read-only fields) then run on the same XCD and meet in
+// its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  constexpr int NX = 8;
+  const int per = (nblocks + NX - 1) / NX;
+  int tall = nblocks % NX;
+  if (tall == 0) tall = NX;
+  const int xcd = bid % NX, local = bid / NX;
+  return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+}
+
 }  // namespace rism

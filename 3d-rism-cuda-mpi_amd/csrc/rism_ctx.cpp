@@ -135,7 +135,7 @@ struct rism_ctx {
   std::vector<double> atoms_host;  // per atom x y z q sig_uv^2[natv] 4eps_uv[natv]
   rism_opts opts{};
   // device
-  double *u = nullptr, *fr = nullptr, *T[3] = {}, *R[3] = {};
+  double *u = nullptr, *fr = nullptr, *T[2] = {}, *Pslot[2] = {}, *R[3] = {};
   cplx *fk = nullptr, *spec = nullptr;
   double *partial = nullptr, *tpartial = nullptr, *sums = nullptr, *scratch = nullptr;
   double *atoms = nullptr, *knots = nullptr, *coef = nullptr, *chitab = nullptr;
@@ -145,7 +145,9 @@ struct rism_ctx {
   // AN2 state (anderson.h:18-40)
   int count = 0, binary = 0;
   double lastx[2] = {0, 0}, lasts[2] = {0, 0};
-  int iT_cur = 0, iT_slot[2] = {1, 2}, iR_cur = 0, iR_slot[2] = {1, 2};
+  // t ping-pongs between T[0] and T[1]; slot p holds P_p = T_p + m R_p in
+  // Pslot[p] and R_p in R[iR_slot[p]] (rotated by pointer)
+  int iT_cur = 0, iR_cur = 0, iR_slot[2] = {1, 2};
   int iT_last = 0, iR_last = 0;
   std::vector<double> hsums;
   bool have_step = false;
@@ -226,6 +228,9 @@ void fill_params(rism_ctx *c) {
   P.dr[2] = c->box[2] / c->nz;
   for (int d = 0; d < 3; ++d) P.kfac[d] = 2.0 * M_PI / c->box[d];
   for (int v = 0; v < RISM_MAX_SITES; ++v) P.qv[v] = v < c->natv ? c->qv[v] : 0.0;
+  // RISM_ZTILE=8 selects the narrower z-pass tiles (tuning experiments)
+  const char *zt = std::getenv("RISM_ZTILE");
+  P.ztile = zt ? std::atoi(zt) : 0;
 }
 
 // chi spline on the knots that bracket the grid's |k| range
@@ -307,8 +312,6 @@ void reset_an2(rism_ctx *c) {
   c->count = c->opts.an2_count;
   c->binary = 0;
   c->iT_cur = 0;
-  c->iT_slot[0] = 1;
-  c->iT_slot[1] = 2;
   c->iR_cur = 0;
   c->iR_slot[0] = 1;
   c->iR_slot[1] = 2;
@@ -460,10 +463,12 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   if ((rc = c->alloc(&c->fr, c->N))) return fail(rc);
   if ((rc = c->alloc(&c->fk, c->N / 2))) return fail(rc);
   if ((rc = c->alloc(&c->spec, nn / 2))) return fail(rc);
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < 2; ++i) {
     if ((rc = c->alloc(&c->T[i], nn))) return fail(rc);
-    if ((rc = c->alloc(&c->R[i], nn))) return fail(rc);
+    if ((rc = c->alloc(&c->Pslot[i], nn))) return fail(rc);
   }
+  for (int i = 0; i < 3; ++i)
+    if ((rc = c->alloc(&c->R[i], nn))) return fail(rc);
   c->zinv_blocks = rism::zinv_num_blocks(c->P);
   if ((rc = c->alloc(&c->partial, c->zinv_blocks * 6))) return fail(rc);
   const int tb = rism::thermo_blocks(c->N);
@@ -485,9 +490,9 @@ int rism_setup_potentials(rism_ctx *c) {
   const size_t nn = c->N * c->natv;
   reset_an2(c);
   // history slots start at zero (the reference leaves them uninitialised)
-  for (int i = 1; i < 3; ++i) {
-    HIPCHECK(hipMemsetAsync(c->T[i], 0, sizeof(double) * nn, c->st));
-    HIPCHECK(hipMemsetAsync(c->R[i], 0, sizeof(double) * nn, c->st));
+  for (int i = 0; i < 2; ++i) {
+    HIPCHECK(hipMemsetAsync(c->Pslot[i], 0, sizeof(double) * nn, c->st));
+    HIPCHECK(hipMemsetAsync(c->R[c->iR_slot[i]], 0, sizeof(double) * nn, c->st));
   }
   if (c->opts.verbose) {
     std::cout << "tabulating solute Lennard-Jones potential ..." << std::endl;
@@ -566,12 +571,12 @@ int rism_anderson(rism_ctx *c) {
     return rism::set_error(RISM_ERR_STATE, "rism_anderson needs a preceding rism_calculate");
   const size_t nn = c->N * c->natv;
   const int b = c->binary;
-  const int newT = c->iT_slot[b], oldR = c->iR_slot[b];
-  double *t = c->T[c->iT_cur], *R = c->R[c->iR_cur];
+  const int oldR = c->iR_slot[b];
+  double *t = c->T[c->iT_cur], *R = c->R[c->iR_cur], *tnew = c->T[1 - c->iT_cur];
   if (c->count > 0) {  // anderson.cu:54-60
     --c->count;
     KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_an2_picard(c->T[newT], t, R, nn, c->st));
+    HIPCHECK(rism::run_an2_picard(tnew, t, R, c->Pslot[b], c->opts.an2_m, nn, c->st));
   } else {  // anderson.cu:61-72, cal_theta anderson.cu:81-102
     double cc[2] = {0, 0}, a[4] = {0, 0, 0, 0};
     for (int v = 0; v < c->natv; ++v) {
@@ -603,15 +608,13 @@ int rism_anderson(rism_ctx *c) {
     c->lasts[0] = s1;
     c->lasts[1] = s2;
     KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_an2_update(c->T[newT], t, R, c->T[c->iT_slot[0]], c->T[c->iT_slot[1]],
-                                  c->R[c->iR_slot[0]], c->R[c->iR_slot[1]], s1, s2,
+    HIPCHECK(rism::run_an2_update(tnew, t, R, c->Pslot[0], c->Pslot[1], c->Pslot[b], s1, s2,
                                   c->opts.an2_m, nn, c->st));
   }
-  // slot[b] <- (t, R) by pointer rotation; the new t is in the retired buffer
-  c->iT_slot[b] = c->iT_cur;
+  // slot[b] <- (t, R): P_b was rewritten by the kernel, R_b by pointer
   c->iR_slot[b] = c->iR_cur;
-  c->iT_cur = newT;
   c->iR_cur = oldR;
+  c->iT_cur = 1 - c->iT_cur;
   c->binary = (b == 0) ? 1 : 0;
   return RISM_OK;
 }

@@ -17,6 +17,7 @@ struct Params {
   int clos;          // 0 KH, 1 HNC
   int mode;          // 0 solver step; 1 plain forward transform; 2 plain inverse
   int shell;         // 1: chi from the cubic-box shell table, 0: spline at |k|
+  int ztile;         // z-pass tile width request (0: default, 8: narrower)
   double dv;         // volume element (forward scale)
   double inv_scale;  // 1 / (dv nx ny (nz/2)): inverse scale of the three passes
   double dr[3];      // grid spacing
@@ -48,11 +49,11 @@ hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const do
                     hipStream_t st);
 hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out,
                            hipStream_t st);
-hipError_t run_an2_picard(double *tnew, const double *t, const double *R, size_t n,
+hipError_t run_an2_picard(double *tnew, const double *t, const double *R, double *Pb, double m,
+                          size_t n, hipStream_t st);
+hipError_t run_an2_update(double *tnew, const double *t, const double *R, const double *P0,
+                          const double *P1, double *Pb, double s1, double s2, double m, size_t n,
                           hipStream_t st);
-hipError_t run_an2_update(double *tnew, const double *t, const double *R, const double *T0,
-                          const double *T1, const double *R0, const double *R1, double s1,
-                          double s2, double m, size_t n, hipStream_t st);
 hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
                            double ubeta, int do_coul, double *u, double *fr, hipStream_t st);
 hipError_t run_fk_half(const double *atoms, int natu, int astride, const Params &P,

@@ -33,27 +33,41 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // AN2 (anderson_cuda.cu:1-22)
 // ---------------------------------------------------------------------------
-// Picard warm-up: slot <- (t, R) is a pointer rotation on the host; the new
-// t = t + R lands in the buffer being retired.
-__global__ void k_an2_picard(double *__restrict__ tnew, const double *__restrict__ t,
-                             const double *__restrict__ R, size_t n) {
+// The reference keeps the slot pairs (T_p, R_p) and forms
+//   u = t + s1 (T0 - t) + s2 (T1 - t)
+//   v = t + R + s1 (T0 + R0 - t - R) + s2 (T1 + R1 - t - R)
+//   t' = u + m (v - u).
+// Writing Q = t + m R and P_p = T_p + m R_p this is
+//   t' = Q + s1 (P0 - Q) + s2 (P1 - Q),
+// so each slot stores P_p (for the update) and R_p (for the Gram matrix,
+// kept by pointer rotation): the update reads t, R, P0, P1 and writes t'
+// and the retired slot's P = Q, 6 words per point instead of 7.
+//
+// Picard warm-up (newdt0): t' = t + R, P_b = t + m R.
+// Both kernels move two consecutive elements per thread as 16-byte accesses
+// (n = natv * N is even: every grid size is even).
+__global__ void k_an2_picard(double2 *__restrict__ tnew, const double2 *__restrict__ t,
+                             const double2 *__restrict__ R, double2 *__restrict__ Pb,
+                             double mcoef, size_t n2) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) tnew[i] = t[i] + R[i];
+  if (i >= n2) return;
+  const double2 ti = t[i], ri = R[i];
+  tnew[i] = make_double2(ti.x + ri.x, ti.y + ri.y);
+  Pb[i] = make_double2(ti.x + mcoef * ri.x, ti.y + mcoef * ri.y);
 }
 
-// tnew aliases T0 or T1 (the slot being retired); each element is read before
+// Pb aliases P0 or P1 (the slot being retired); each element is read before
 // it is written by the same thread.
-__global__ void k_an2_update(double *tnew, const double *__restrict__ t,
-                             const double *__restrict__ R, const double *T0, const double *T1,
-                             const double *__restrict__ R0, const double *__restrict__ R1,
-                             double s1, double s2, double mcoef, size_t n) {
+__global__ void k_an2_update(double2 *__restrict__ tnew, const double2 *__restrict__ t,
+                             const double2 *__restrict__ R, const double2 *P0, const double2 *P1,
+                             double2 *Pb, double s1, double s2, double mcoef, size_t n2) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double ti = t[i], ri = R[i];
-  const double t0 = T0[i], t1 = T1[i];
-  const double uu = ti + s1 * (t0 - ti) + s2 * (t1 - ti);
-  const double vv = ti + ri + s1 * (t0 + R0[i] - ti - ri) + s2 * (t1 + R1[i] - ti - ri);
-  tnew[i] = uu + mcoef * (vv - uu);
+  if (i >= n2) return;
+  const double2 ti = t[i], ri = R[i], p0 = P0[i], p1 = P1[i];
+  const double qx = ti.x + mcoef * ri.x, qy = ti.y + mcoef * ri.y;
+  tnew[i] = make_double2(qx + s1 * (p0.x - qx) + s2 * (p1.x - qx),
+                         qy + s1 * (p0.y - qy) + s2 * (p1.y - qy));
+  Pb[i] = make_double2(qx, qy);
 }
 
 // ---------------------------------------------------------------------------
@@ -328,17 +342,23 @@ hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out
 }
 
 
-hipError_t run_an2_picard(double *tnew, const double *t, const double *R, size_t n,
-                          hipStream_t st) {
-  hipLaunchKernelGGL(k_an2_picard, dim3(nblk(n, 256)), dim3(256), 0, st, tnew, t, R, n);
+hipError_t run_an2_picard(double *tnew, const double *t, const double *R, double *Pb, double m,
+                          size_t n, hipStream_t st) {
+  if (n & 1) return hipErrorInvalidValue;
+  const size_t n2 = n / 2;
+  hipLaunchKernelGGL(k_an2_picard, dim3(nblk(n2, 256)), dim3(256), 0, st, (double2 *)tnew,
+                     (const double2 *)t, (const double2 *)R, (double2 *)Pb, m, n2);
   return hipGetLastError();
 }
 
-hipError_t run_an2_update(double *tnew, const double *t, const double *R, const double *T0,
-                          const double *T1, const double *R0, const double *R1, double s1,
-                          double s2, double m, size_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_an2_update, dim3(nblk(n, 256)), dim3(256), 0, st, tnew, t, R, T0, T1, R0,
-                     R1, s1, s2, m, n);
+hipError_t run_an2_update(double *tnew, const double *t, const double *R, const double *P0,
+                          const double *P1, double *Pb, double s1, double s2, double m, size_t n,
+                          hipStream_t st) {
+  if (n & 1) return hipErrorInvalidValue;
+  const size_t n2 = n / 2;
+  hipLaunchKernelGGL(k_an2_update, dim3(nblk(n2, 256)), dim3(256), 0, st, (double2 *)tnew,
+                     (const double2 *)t, (const double2 *)R, (const double2 *)P0,
+                     (const double2 *)P1, (double2 *)Pb, s1, s2, m, n2);
   return hipGetLastError();
 }
 

@@ -34,14 +34,11 @@ namespace rism {
 // ---------------------------------------------------------------------------
 // 3. DFT along x, OZ with chi_vv(|k|), inverse DFT along x
 // ---------------------------------------------------------------------------
-// One workgroup per (m, y) row, all sites: natv rows of nx complex in LDS;
-// natv * (nx / max radix) <= 512 threads, natv <= 8.
-// chi_vv is evaluated from the natural cubic spline of the .xvk table at
-// |k| = sqrt(kx^2 + ky^2 + kz^2) (solvent_spline.cu:44-55 evaluates the same
-// spline at the same |k| shell values, splint.cc:4-20 picks the interval).
 // chi_vv(|k|) sources for the OZ step.  ChiTable: the cubic-box shell table
 // (one row of natv^2 values per k^2 shell, built once by k_chi_table);
-// ChiSpline: the spline evaluated at |k| directly (any box).
+// ChiSpline: the natural cubic spline of the .xvk table evaluated at |k|
+// (solvent_spline.cu:44-55 evaluates the same spline at the same |k| shell
+// values; splint.cc:4-20 picks the interval).
 struct ChiTable {
   const double *row;
   __device__ __forceinline__ double operator()(int pw) const { return row[pw]; }
@@ -55,57 +52,83 @@ struct ChiSpline {
   }
 };
 
-// OZ at the mirror pair of points p1 (kx) and p2 (-kx) of one row, which
-// share |k| and therefore chi: H_v = sum_w X[v][w] (C_w - q_w fk)
-// (mqvfk + oz, calculate_cuda.cu:32-52).  NVM is a compile-time bound on the
-// site count so the per-site values stay in registers.
-template <int NVM, class Chi>
-__device__ __forceinline__ void oz_pair(cplx *lds, int CS, int p1, int p2, cplx fk1, cplx fk2,
-                                        const Params &P, int nv, const Chi &X) {
-  cplx c1[NVM], c2[NVM];
+// OZ at NP points that share |k| and therefore chi (here (kx, ky) and
+// (-kx, ky) of one row): H_v = sum_w X[v][w] (C_w - q_w fk)
+// (mqvfk + oz, calculate_cuda.cu:32-52).  off[k] is the LDS offset of point
+// k in site 0's column; site w sits w*CS further.  NVM is a compile-time
+// bound on the site count so the per-site values stay in registers.
+template <int NVM, int NP, class Chi>
+__device__ __forceinline__ void oz_points(cplx *lds, int CS, const int (&off)[NP],
+                                          const cplx (&fk)[NP], const Params &P, int nv,
+                                          const Chi &X) {
+  cplx c[NP][NVM];
 #pragma unroll
   for (int w = 0; w < NVM; ++w) {
     if (w < nv) {
-      c1[w] = lds[w * CS + p1] - P.qv[w] * fk1;
-      c2[w] = lds[w * CS + p2] - P.qv[w] * fk2;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) c[k][w] = lds[off[k] + w * CS] - P.qv[w] * fk[k];
     }
   }
   // one output site at a time keeps the live chi values to one row of NVM
 #pragma unroll 1
   for (int v = 0; v < NVM; ++v) {
     if (v < nv) {
-      cplx a1 = mk(0.0, 0.0), a2 = mk(0.0, 0.0);
+      cplx a[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) a[k] = mk(0.0, 0.0);
 #pragma unroll
       for (int w = 0; w < NVM; ++w) {
         if (w < nv) {
           const double x = X(v * nv + w);
-          a1 = a1 + x * c1[w];
-          a2 = a2 + x * c2[w];
+#pragma unroll
+          for (int k = 0; k < NP; ++k) a[k] = a[k] + x * c[k][w];
         }
       }
-      lds[v * CS + p1] = a1;
-      lds[v * CS + p2] = a2;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) lds[off[k] + v * CS] = a[k];
     }
   }
 }
 
-template <class Chi>
-__device__ __forceinline__ void oz_dispatch(cplx *lds, int CS, int p1, int p2, cplx fk1,
-                                            cplx fk2, const Params &P, const Chi &X) {
+// NVMAX (4 or 8) bounds the site counts compiled into one kernel: the
+// register allocation follows the largest case, so natv <= 4 gets a kernel
+// without the 5..8-site paths.
+template <int NVMAX, int NP, class Chi>
+__device__ __forceinline__ void oz_dispatch(cplx *lds, int CS, const int (&off)[NP],
+                                            const cplx (&fk)[NP], const Params &P,
+                                            const Chi &X) {
   switch (P.natv) {
-    case 1: oz_pair<1>(lds, CS, p1, p2, fk1, fk2, P, 1, X); break;
-    case 2: oz_pair<2>(lds, CS, p1, p2, fk1, fk2, P, 2, X); break;
-    case 3: oz_pair<3>(lds, CS, p1, p2, fk1, fk2, P, 3, X); break;
-    case 4: oz_pair<4>(lds, CS, p1, p2, fk1, fk2, P, 4, X); break;
-    case 5: oz_pair<5>(lds, CS, p1, p2, fk1, fk2, P, 5, X); break;
-    case 6: oz_pair<6>(lds, CS, p1, p2, fk1, fk2, P, 6, X); break;
-    case 7: oz_pair<7>(lds, CS, p1, p2, fk1, fk2, P, 7, X); break;
-    case 8: oz_pair<8>(lds, CS, p1, p2, fk1, fk2, P, 8, X); break;
-    default: oz_pair<8>(lds, CS, p1, p2, fk1, fk2, P, 8, X); break;
+    case 1: oz_points<1>(lds, CS, off, fk, P, 1, X); break;
+    case 2: oz_points<2>(lds, CS, off, fk, P, 2, X); break;
+    case 3: oz_points<3>(lds, CS, off, fk, P, 3, X); break;
+    case 4: oz_points<4>(lds, CS, off, fk, P, 4, X); break;
+    default:
+      if constexpr (NVMAX > 4) {
+        switch (P.natv) {
+          case 5: oz_points<5>(lds, CS, off, fk, P, 5, X); break;
+          case 6: oz_points<6>(lds, CS, off, fk, P, 6, X); break;
+          case 7: oz_points<7>(lds, CS, off, fk, P, 7, X); break;
+          default: oz_points<8>(lds, CS, off, fk, P, 8, X); break;
+        }
+      }
+      break;
   }
 }
 
-template <int L>
+template <int NVMAX, class Chi>
+__device__ __forceinline__ void oz_row(cplx *lds, int CS, int jx, int L, const cplx *fkr,
+                                       const Params &P, const Chi &X) {
+  const int jm = L - 1 - jx;
+  const int off[2] = {lpad(jx), lpad(jm)};
+  const cplx fk[2] = {fkr[jx], fkr[jm]};
+  oz_dispatch<NVMAX, 2>(lds, CS, off, fk, P, X);
+}
+
+// One workgroup per (m, y) row, all sites: natv columns of nx complex in LDS,
+// natv * T threads.  SHELL selects the chi source at compile time (the
+// cubic-box shell table or the spline at |k|) and NVMAX bounds the site
+// count, so each instantiation carries only the registers it needs.
+template <int L, int NVMAX, bool SHELL>
 __global__ void __launch_bounds__(512)
     k_xpass_oz(cplx *__restrict__ spec, const cplx *__restrict__ fkraw, Params P,
                const cplx *__restrict__ twL, const cplx *__restrict__ prex,
@@ -127,27 +150,24 @@ __global__ void __launch_bounds__(512)
   auto nostore = [&](int, cplx) {};
   fft_column<L, -1, true, false>(mylds, j, twL, load, nostore);
 
-  // OZ at every k of the row, two mirror points (kx, -kx) per iteration
+  // OZ at every k of the row, the mirror points (kx, -kx) together
   const int nthr = nv * T;
   const int nv2 = nv * nv;
-  const double ky = P.kfac[1] * ((double)y - (double)P.ny / 2.0 + 0.5);
-  const double kz = P.kfac[2] * ((double)(m + P.nz / 2) - (double)P.nz / 2.0 + 0.5);
-  const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates
-  const long syz = oy * oy + oz * oz;
+  const cplx *fkr = fkraw + rowoff;
   for (int jx = tid; jx < L / 2; jx += nthr) {
-    const int jm = L - 1 - jx;
-    const cplx fk1 = fkraw[rowoff + jx], fk2 = fkraw[rowoff + jm];
-    const int p1 = lpad(jx), p2 = lpad(jm);
-    if (P.shell) {
+    if constexpr (SHELL) {
+      const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates
       const long ox = 2 * jx - L + 1;
-      const long sidx = (ox * ox + syz - 3) >> 3;  // |k|^2 = (pi/L)^2 (8 sidx + 3)
-      oz_dispatch(lds, CS, p1, p2, fk1, fk2, P, ChiTable{chitab + sidx * nv2});
+      const long sidx = (ox * ox + oy * oy + oz * oz - 3) >> 3;  // |k|^2 = (pi/L)^2 (8 sidx + 3)
+      oz_row<NVMAX>(lds, CS, jx, L, fkr, P, ChiTable{chitab + sidx * nv2});
     } else {
+      const double ky = P.kfac[1] * ((double)y - (double)P.ny / 2.0 + 0.5);
+      const double kz = P.kfac[2] * ((double)(m + P.nz / 2) - (double)P.nz / 2.0 + 0.5);
       const double kx = P.kfac[0] * ((double)jx - (double)L / 2.0 + 0.5);
       const double kk = sqrt(kx * kx + ky * ky + kz * kz);
       const int ii = spline_interval(knots, P.np, P.x0, P.idk, kk);
-      oz_dispatch(lds, CS, p1, p2, fk1, fk2, P,
-                  ChiSpline{coef + (size_t)ii * 4 * nv2, kk - knots[ii], nv2});
+      oz_row<NVMAX>(lds, CS, jx, L, fkr, P,
+                    ChiSpline{coef + (size_t)ii * 4 * nv2, kk - knots[ii], nv2});
     }
   }
   __syncthreads();
@@ -188,19 +208,30 @@ template <int L>
 static void launch_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *twL,
                        const cplx *prex, const double *knots, const double *coef,
                        const double *chitab, hipStream_t st) {
+  if (P.mode == 1 || P.mode == 2) {
+    dim3 grid(P.ny, P.nz / 2);
+    const int threads = P.natv * FftShape<L>::T;
+    const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cplx);
+    if (P.mode == 1)
+      hipLaunchKernelGGL((k_xpass_plain<L, -1>), grid, dim3(threads), lds, st, spec, P, twL, prex);
+    else
+      hipLaunchKernelGGL((k_xpass_plain<L, 1>), grid, dim3(threads), lds, st, spec, P, twL, prex);
+    return;
+  }
   dim3 grid(P.ny, P.nz / 2);
   const int threads = P.natv * FftShape<L>::T;
   const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cplx);
-  if (lds > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xpass_oz<L>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (P.mode == 1)
-    hipLaunchKernelGGL((k_xpass_plain<L, -1>), grid, dim3(threads), lds, st, spec, P, twL, prex);
-  else if (P.mode == 2)
-    hipLaunchKernelGGL((k_xpass_plain<L, 1>), grid, dim3(threads), lds, st, spec, P, twL, prex);
+  auto go = [&](auto kern) {
+    if (lds > 65536)
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, grid, dim3(threads), lds, st, spec, fkraw, P, twL, prex, knots, coef,
+                       chitab);
+  };
+  if (P.natv <= 4)
+    P.shell ? go(k_xpass_oz<L, 4, true>) : go(k_xpass_oz<L, 4, false>);
   else
-    hipLaunchKernelGGL((k_xpass_oz<L>), grid, dim3(threads), lds, st, spec, fkraw, P, twL, prex,
-                       knots, coef, chitab);
+    P.shell ? go(k_xpass_oz<L, 8, true>) : go(k_xpass_oz<L, 8, false>);
 }
 int xoz_threads(int L, int natv) {
   switch (L) {

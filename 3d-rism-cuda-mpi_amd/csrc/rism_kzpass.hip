@@ -24,9 +24,14 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   __shared__ cplx lds[B * CS];
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
-  const int x = blockIdx.x * B + c;
-  const int y = blockIdx.y;
-  const int v = blockIdx.z;
+  // logical workgroup order: site fastest, then x tile, then y (sites of
+  // one tile share fr in L2)
+  const int ntx = (P.nx + B - 1) / B;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int v = lb % P.natv;
+  const int tx = (lb / P.natv) % ntx;
+  const int y = lb / (P.natv * ntx);
+  const int x = tx * B + c;
   const bool ok = x < P.nx;
   const size_t plane = (size_t)P.nx * P.ny;
   const size_t N = plane * (size_t)P.nz;
@@ -80,9 +85,12 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   __shared__ cplx lds[B * CS];
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
-  const int x = blockIdx.x * B + c;
-  const int y = blockIdx.y;
-  const int v = blockIdx.z;
+  const int ntx = (P.nx + B - 1) / B;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int v = lb % P.natv;
+  const int tx = (lb / P.natv) % ntx;
+  const int y = lb / (P.natv * ntx);
+  const int x = tx * B + c;
   const bool ok = x < P.nx;
   const size_t plane = (size_t)P.nx * P.ny;
   const size_t N = plane * (size_t)P.nz;
@@ -168,29 +176,48 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   if (tid < 6) {
     double a = 0.0;
     for (int w = 0; w < nparts; ++w) a += red[w * 6 + tid];
-    const size_t blk = ((size_t)v * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const size_t blk = (size_t)v * ntx * P.ny + (size_t)y * ntx + tx;  // [site][tile]
     partial[blk * 6 + tid] = a;
   }
 }
 
+// Tile width of the z passes: the default of tile_cols<M>(), or half of it
+// when P.ztile asks for narrower tiles (more workgroups per CU).
+template <int M>
+constexpr int ztile_narrow() {
+  return tile_cols<M>() >= 8 ? tile_cols<M>() / 2 : tile_cols<M>();
+}
+template <int M>
+static int ztile_of(const Params &P) {
+  return (P.ztile > 0 && P.ztile < tile_cols<M>()) ? ztile_narrow<M>() : tile_cols<M>();
+}
+static inline unsigned zgrid(const Params &P, int B) {
+  return (unsigned)(((P.nx + B - 1) / B) * P.ny * P.natv);
+}
 template <int M>
 static void launch_zfwd(const Params &P, const double *t, const double *u, const double *fr,
                         cplx *spec, const cplx *twM, const cplx *zpre, const cplx *zpost,
                         hipStream_t st) {
-  constexpr int B = tile_cols<M>();
-  dim3 grid((P.nx + B - 1) / B, P.ny, P.natv);
-  hipLaunchKernelGGL((k_zfwd_closure<M, B>), grid, dim3(B * FftShape<M>::T), 0, st, t, u, fr,
-                     P, spec, twM, zpre, zpost);
+  constexpr int B = tile_cols<M>(), B2 = ztile_narrow<M>();
+  if (ztile_of<M>(P) == B)
+    hipLaunchKernelGGL((k_zfwd_closure<M, B>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0,
+                       st, t, u, fr, P, spec, twM, zpre, zpost);
+  else
+    hipLaunchKernelGGL((k_zfwd_closure<M, B2>), dim3(zgrid(P, B2)), dim3(B2 * FftShape<M>::T),
+                       0, st, t, u, fr, P, spec, twM, zpre, zpost);
 }
 template <int M>
 static void launch_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
                         double *R, const double *R0, const double *R1, double *partial,
                         const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,
                         hipStream_t st) {
-  constexpr int B = tile_cols<M>();
-  dim3 grid((P.nx + B - 1) / B, P.ny, P.natv);
-  hipLaunchKernelGGL((k_zinv_resid<M, B>), grid, dim3(B * FftShape<M>::T), 0, st, spec, t, u,
-                     R, R0, R1, P, partial, twM, zpre, zpost, use_hist);
+  constexpr int B = tile_cols<M>(), B2 = ztile_narrow<M>();
+  if (ztile_of<M>(P) == B)
+    hipLaunchKernelGGL((k_zinv_resid<M, B>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0, st,
+                       spec, t, u, R, R0, R1, P, partial, twM, zpre, zpost, use_hist);
+  else
+    hipLaunchKernelGGL((k_zinv_resid<M, B2>), dim3(zgrid(P, B2)), dim3(B2 * FftShape<M>::T), 0,
+                       st, spec, t, u, R, R0, R1, P, partial, twM, zpre, zpost, use_hist);
 }
 bool fft_length_supported(int L) {
   switch (L) {
@@ -208,7 +235,7 @@ size_t zinv_num_blocks(const Params &P) {
   switch (M) {
 #define X(n) \
   case n:    \
-    B = tile_cols<n>(); \
+    B = ztile_of<n>(P); \
     break;
     RISM_LENGTHS(X)
 #undef X

@@ -153,6 +153,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--maxstep", type=int, default=10000,
+                    help="iteration cap (profiling runs only; the metric needs convergence)")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of GIB GiB five times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs one solve per GPU and has no "
@@ -162,6 +167,15 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration > 0:
+        n = int(args.copy_calibration * (1 << 30)) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 5}), flush=True)
+        return
 
     rism, grid, box, solv, sol, xvk, desc = workload(args.workload)
     solu = {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"], "uxyz": sol["xyz"]}
@@ -169,14 +183,14 @@ def main():
         f"natv {solv['natv']}")
     g = rism.RISM3D(grid, box, solv, solu, rism.KH, 2, 0.6, 0.2)
     for i in range(args.warmup):
-        th, tm = g.run(maxstep=10000, tol=1e-6)
+        th, tm = g.run(maxstep=args.maxstep, tol=1e-6)
         log(f"[bench] warmup {i}: {tm}")
     g.kernel_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     results = []
     for i in range(args.steps):
-        th, tm = g.run(maxstep=10000, tol=1e-6)
+        th, tm = g.run(maxstep=args.maxstep, tol=1e-6)
         results.append((th, tm))
         log(f"[bench] step {i}: {tm} SFE_SC {th['SFE_SC']:.5f}")
     torch.cuda.synchronize()
