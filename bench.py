@@ -74,13 +74,23 @@ def workload(name):
     return rism, grid, box, solv, sol, xvk, desc
 
 
+# HBM bytes per launch of the hot-loop kernels at cfg2 (512^3, 4 sites, FP64),
+# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one counter per
+# run) and scaled by the same counters on a plain 2 GiB device copy
+# (fetch x2, write x1).
+MEASURED_TRAFFIC_CFG2 = {
+    "zfwd_closure": 13.97e9, "ypass_fwd": 8.59e9, "xpass_oz": 12.50e9, "ypass_inv": 8.59e9,
+    "zinv_resid": 24.35e9, "an2_update": 25.77e9,
+}
+
+
 def algorithmic_bytes(natv, N, hist=True):
     """Minimal HBM bytes per launch of each hot-loop kernel (FP64 words of 8 B):
     zfwd reads t, u (2n) and fr (1), writes the half spectrum (n);
     each y pass reads and writes the spectrum (2n);
     xpass_oz reads the spectrum (n) and the half fk (1), writes (n);
     zinv reads the spectrum, t, u (3n) and R0, R1 (2n), writes R (n);
-    an2_update reads t, R, T0, T1, R0, R1 (6n), writes t (n)."""
+    an2_update reads t, R, P0, P1 (4n), writes t', P_b (2n)."""
     w = 8.0 * N
     n = natv
     return {
@@ -89,7 +99,7 @@ def algorithmic_bytes(natv, N, hist=True):
         "xpass_oz": (2 * n + 1) * w,
         "ypass_inv": 2 * n * w,
         "zinv_resid": (6 * n if hist else 4 * n) * w,
-        "an2_update": 7 * n * w,
+        "an2_update": 6 * n * w,
     }
 
 
@@ -255,7 +265,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC_CFG2.get(dom) if args.workload == "cfg2" else None,
             "algorithmic_bytes_per_launch": ab[dom],
             "avg_launch_ms": avg_s * 1000.0,
         },
