@@ -29,6 +29,8 @@ const double kAvogadro = 6.02214076e23;          // physical.h:5
 const double kBohr = 0.5291771500001;            // physical.h:8
 const double kBoltzmann = 1.380649e-23;          // physical.h:11
 const double kHartree = 4.359744722207185e-18;   // physical.h:14
+// width of the per-workgroup partial sums of the residual pass
+const int kSumWidth = 1 + RISM_MAX_HIST;
 
 #define HIPCHECK(expr)                                                                \
   do {                                                                                \
@@ -148,7 +150,20 @@ struct rism_ctx {
   // t ping-pongs between T[0] and T[1]; slot p holds P_p = T_p + m R_p in
   // Pslot[p] and R_p in R[iR_slot[p]] (rotated by pointer)
   int iT_cur = 0, iR_cur = 0, iR_slot[2] = {1, 2};
-  int iT_last = 0, iR_last = 0;
+  int iT_last = 0;
+  double *Rlast = nullptr;  // residual written by the last rism_calculate
+  bool fresh = false;       // no rism_anderson since that rism_calculate
+  // Anderson(m) state (opts.anderson_m > 0): residuals in Rb[m + 1], mixed
+  // iterates P_p = t_p + beta R_p in Pb[m]; hist lists the (R, P) buffer
+  // pairs oldest first; D holds the weighted dots <R_a, R_b> by R buffer.
+  struct HEnt {
+    int rb, pb;
+  };
+  std::vector<double *> Rb, Pb;
+  std::vector<HEnt> hist;
+  int cur_rb = 0, next_pb = 0;
+  std::vector<double> D;
+  std::vector<double> lastth;
   std::vector<double> hsums;
   bool have_step = false;
   rism_timing timing{};
@@ -228,9 +243,6 @@ void fill_params(rism_ctx *c) {
   P.dr[2] = c->box[2] / c->nz;
   for (int d = 0; d < 3; ++d) P.kfac[d] = 2.0 * M_PI / c->box[d];
   for (int v = 0; v < RISM_MAX_SITES; ++v) P.qv[v] = v < c->natv ? c->qv[v] : 0.0;
-  // RISM_ZTILE=8 selects the narrower z-pass tiles (tuning experiments)
-  const char *zt = std::getenv("RISM_ZTILE");
-  P.ztile = zt ? std::atoi(zt) : 0;
 }
 
 // chi spline on the knots that bracket the grid's |k| range
@@ -316,8 +328,13 @@ void reset_an2(rism_ctx *c) {
   c->iR_slot[0] = 1;
   c->iR_slot[1] = 2;
   c->iT_last = 0;
-  c->iR_last = 0;
+  c->Rlast = nullptr;
+  c->fresh = false;
   c->have_step = false;
+  c->hist.clear();
+  c->cur_rb = 0;
+  c->next_pb = 0;
+  c->lastth.clear();
   c->lastx[0] = c->lastx[1] = c->lasts[0] = c->lasts[1] = 0.0;
 }
 
@@ -368,6 +385,120 @@ struct KTime {
   }
 };
 
+// Coefficients of the Anderson(m) step from the weighted dots D: minimise
+// |R - sum_p th_p (R - R_p)| over the newest k history entries, i.e. solve
+// G th = b with G_pq = <R - R_p, R - R_q>, b_p = <R, R - R_p>, after
+// scaling G to unit diagonal, by Gaussian elimination with partial
+// pivoting.  When a pivot falls below 1e-12 (or a diagonal is not
+// positive) the oldest entry is left out and the solve repeated.
+// Returns th for all entries (zeros for the ones left out).
+std::vector<double> anderson_m_coeffs(const std::vector<double> &D, int m1, int cur,
+                                      const std::vector<int> &rb) {
+  const int nh = (int)rb.size();
+  std::vector<double> th(nh, 0.0);
+  const double rr = D[(size_t)cur * m1 + cur];
+  for (int first = 0; first < nh; ++first) {
+    const int k = nh - first;
+    std::vector<double> G((size_t)k * k), b(k), sc(k);
+    bool ok = true;
+    for (int p = 0; p < k && ok; ++p) {
+      const int bp = rb[first + p];
+      const double dp = D[(size_t)cur * m1 + bp];
+      b[p] = rr - dp;
+      for (int q = 0; q < k; ++q) {
+        const int bq = rb[first + q];
+        G[(size_t)p * k + q] = rr - dp - D[(size_t)cur * m1 + bq] + D[(size_t)bp * m1 + bq];
+      }
+      ok = G[(size_t)p * k + p] > 0.0;
+      if (ok) sc[p] = 1.0 / std::sqrt(G[(size_t)p * k + p]);
+    }
+    if (!ok) continue;
+    for (int p = 0; p < k; ++p) {
+      b[p] *= sc[p];
+      for (int q = 0; q < k; ++q) G[(size_t)p * k + q] *= sc[p] * sc[q];
+    }
+    std::vector<int> perm(k);
+    for (int p = 0; p < k; ++p) perm[p] = p;
+    for (int col = 0; col < k && ok; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < k; ++r)
+        if (std::fabs(G[(size_t)r * k + col]) > std::fabs(G[(size_t)piv * k + col])) piv = r;
+      if (std::fabs(G[(size_t)piv * k + col]) < 1e-12) {
+        ok = false;
+        break;
+      }
+      if (piv != col) {
+        for (int q = 0; q < k; ++q) std::swap(G[(size_t)piv * k + q], G[(size_t)col * k + q]);
+        std::swap(b[piv], b[col]);
+      }
+      for (int r = col + 1; r < k; ++r) {
+        const double f = G[(size_t)r * k + col] / G[(size_t)col * k + col];
+        for (int q = col; q < k; ++q) G[(size_t)r * k + q] -= f * G[(size_t)col * k + q];
+        b[r] -= f * b[col];
+      }
+    }
+    if (!ok) continue;
+    std::vector<double> x(k);
+    for (int r = k - 1; r >= 0; --r) {
+      double a = b[r];
+      for (int q = r + 1; q < k; ++q) a -= G[(size_t)r * k + q] * x[q];
+      x[r] = a / G[(size_t)r * k + r];
+    }
+    for (int p = 0; p < k; ++p) th[first + p] = x[p] * sc[p];
+    return th;
+  }
+  return th;
+}
+
+// One Anderson(m) step: the first an2_count steps are the reference's
+// Picard steps t' = t + R (anderson.cu:54-60); after that
+//   t' = Q + sum_p th_p (P_p - Q),  Q = t + beta R,  beta = an2_m.
+// Either way the new pair (R, Q) enters the history, retiring the oldest
+// pair once m are held.
+int anderson_m_step(rism_ctx *c) {
+  const size_t nn = c->N * c->natv;
+  const int m = c->opts.anderson_m, m1 = m + 1;
+  double *t = c->T[c->iT_cur], *tnew = c->T[1 - c->iT_cur];
+  double *R = c->Rb[c->cur_rb];
+  // slot for the new pair: the oldest when full, else a fresh one
+  const bool full = (int)c->hist.size() == m;
+  const rism_ctx::HEnt old = full ? c->hist.front() : rism_ctx::HEnt{-1, c->next_pb};
+  double *Pdst = c->Pb[old.pb];
+  if (c->count > 0) {
+    --c->count;
+    KTime kt(c, RISM_K_AN2);
+    HIPCHECK(rism::run_an2_picard(tnew, t, R, Pdst, c->opts.an2_m, nn, c->st));
+    c->lastth.clear();
+  } else {
+    std::vector<int> rb;
+    for (const auto &h : c->hist) rb.push_back(h.rb);
+    c->lastth = anderson_m_coeffs(c->D, m1, c->cur_rb, rb);
+    rism::HistUpdate H{};
+    H.nh = (int)c->hist.size();
+    for (int p = 0; p < H.nh; ++p) {
+      H.p[p] = c->Pb[c->hist[p].pb];
+      H.x[p] = c->lastth[p];
+    }
+    KTime kt(c, RISM_K_AN2);
+    HIPCHECK(rism::run_andm_update(tnew, t, R, H, Pdst, c->opts.an2_m, nn, c->st));
+  }
+  if (full) c->hist.erase(c->hist.begin());
+  else ++c->next_pb;
+  c->hist.push_back({c->cur_rb, old.pb});
+  // next residual buffer: the retired one, else the lowest free index
+  if (full) {
+    c->cur_rb = old.rb;
+  } else {
+    std::vector<char> used(m1, 0);
+    for (const auto &h : c->hist) used[h.rb] = 1;
+    int f = 0;
+    while (used[f]) ++f;
+    c->cur_rb = f;
+  }
+  c->iT_cur = 1 - c->iT_cur;
+  return RISM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -408,6 +539,10 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   }
   if (opts->closure != RISM_CLOSURE_KH && opts->closure != RISM_CLOSURE_HNC) {
     *err = rism::set_error(RISM_ERR_CLOSURE, "Error: Unexpected closure switch ");
+    return nullptr;
+  }
+  if (opts->anderson_m < 0 || opts->anderson_m > RISM_MAX_HIST) {
+    *err = rism::set_error(RISM_ERR_ARG, "anderson_m must be in [0, 16]");
     return nullptr;
   }
   rism_ctx *c = new rism_ctx();
@@ -463,19 +598,30 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   if ((rc = c->alloc(&c->fr, c->N))) return fail(rc);
   if ((rc = c->alloc(&c->fk, c->N / 2))) return fail(rc);
   if ((rc = c->alloc(&c->spec, nn / 2))) return fail(rc);
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < 2; ++i)
     if ((rc = c->alloc(&c->T[i], nn))) return fail(rc);
-    if ((rc = c->alloc(&c->Pslot[i], nn))) return fail(rc);
+  const int am = c->opts.anderson_m;
+  if (am == 0) {
+    for (int i = 0; i < 2; ++i)
+      if ((rc = c->alloc(&c->Pslot[i], nn))) return fail(rc);
+    for (int i = 0; i < 3; ++i)
+      if ((rc = c->alloc(&c->R[i], nn))) return fail(rc);
+  } else {
+    c->Rb.assign(am + 1, nullptr);
+    c->Pb.assign(am, nullptr);
+    for (auto &b : c->Rb)
+      if ((rc = c->alloc(&b, nn))) return fail(rc);
+    for (auto &b : c->Pb)
+      if ((rc = c->alloc(&b, nn))) return fail(rc);
+    c->D.assign((size_t)(am + 1) * (am + 1), 0.0);
   }
-  for (int i = 0; i < 3; ++i)
-    if ((rc = c->alloc(&c->R[i], nn))) return fail(rc);
   c->zinv_blocks = rism::zinv_num_blocks(c->P);
-  if ((rc = c->alloc(&c->partial, c->zinv_blocks * 6))) return fail(rc);
+  if ((rc = c->alloc(&c->partial, c->zinv_blocks * kSumWidth))) return fail(rc);
   const int tb = rism::thermo_blocks(c->N);
   if ((rc = c->alloc(&c->tpartial, (size_t)tb * 4 * c->natv))) return fail(rc);
-  if ((rc = c->alloc(&c->sums, 6 * RISM_MAX_SITES))) return fail(rc);
+  if ((rc = c->alloc(&c->sums, kSumWidth * RISM_MAX_SITES))) return fail(rc);
   if ((rc = c->alloc(&c->scratch, c->N))) return fail(rc);
-  c->hsums.assign(6 * c->natv, 0.0);
+  c->hsums.assign(kSumWidth * c->natv, 0.0);
   reset_an2(c);
   if (hipMemsetAsync(c->T[0], 0, sizeof(double) * nn, c->st) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
@@ -490,9 +636,11 @@ int rism_setup_potentials(rism_ctx *c) {
   const size_t nn = c->N * c->natv;
   reset_an2(c);
   // history slots start at zero (the reference leaves them uninitialised)
-  for (int i = 0; i < 2; ++i) {
-    HIPCHECK(hipMemsetAsync(c->Pslot[i], 0, sizeof(double) * nn, c->st));
-    HIPCHECK(hipMemsetAsync(c->R[c->iR_slot[i]], 0, sizeof(double) * nn, c->st));
+  if (c->opts.anderson_m == 0) {
+    for (int i = 0; i < 2; ++i) {
+      HIPCHECK(hipMemsetAsync(c->Pslot[i], 0, sizeof(double) * nn, c->st));
+      HIPCHECK(hipMemsetAsync(c->R[c->iR_slot[i]], 0, sizeof(double) * nn, c->st));
+    }
   }
   if (c->opts.verbose) {
     std::cout << "tabulating solute Lennard-Jones potential ..." << std::endl;
@@ -539,27 +687,52 @@ int rism_calculate(rism_ctx *c, double *rms) {
     KTime kt(c, RISM_K_YINV);
     HIPCHECK(rism::run_y(P, c->spec, +1, c->tb.twy, c->tb.prey, c->st));
   }
-  const int use_hist = (c->count == 0);
+  const bool andm = c->opts.anderson_m > 0;
+  const int W = andm ? kSumWidth : 6;
+  double *Rout = andm ? c->Rb[c->cur_rb] : c->R[c->iR_cur];
   {
     KTime kt(c, RISM_K_ZINV);
-    HIPCHECK(rism::run_zinv(P, c->spec, t, c->u, c->R[c->iR_cur], c->R[c->iR_slot[0]],
-                            c->R[c->iR_slot[1]], c->partial, c->tb.twz, c->tb.zpre, c->tb.zpost,
-                            use_hist, c->st));
+    if (andm) {
+      rism::Hist H{};
+      H.nh = (int)c->hist.size();
+      for (int p = 0; p < H.nh; ++p) H.r[p] = c->Rb[c->hist[p].rb];
+      HIPCHECK(rism::run_zinv_hist(P, c->spec, t, c->u, Rout, H, c->partial, c->tb.twz,
+                                   c->tb.zpre, c->tb.zpost, c->st));
+    } else {
+      const int use_hist = (c->count == 0);
+      HIPCHECK(rism::run_zinv(P, c->spec, t, c->u, Rout, c->R[c->iR_slot[0]],
+                              c->R[c->iR_slot[1]], c->partial, c->tb.twz, c->tb.zpre,
+                              c->tb.zpost, use_hist, c->st));
+    }
   }
   const size_t nb = c->zinv_blocks / c->natv;
   {
     KTime kt(c, RISM_K_REDUCE);
     for (int v = 0; v < c->natv; ++v)
-      HIPCHECK(rism::run_reduce_cols(c->partial + (size_t)v * nb * 6, nb, 6, c->sums + 6 * v, c->st));
+      HIPCHECK(rism::run_reduce_cols(c->partial + (size_t)v * nb * W, nb, W, c->sums + W * v, c->st));
   }
-  HIPCHECK(hipMemcpyAsync(c->hsums.data(), c->sums, sizeof(double) * 6 * c->natv,
+  HIPCHECK(hipMemcpyAsync(c->hsums.data(), c->sums, sizeof(double) * W * c->natv,
                           hipMemcpyDeviceToHost, c->st));
   HIPCHECK(hipStreamSynchronize(c->st));
   if (c->ktime) ktime_resolve(c);
   double ss = 0.0;
-  for (int v = 0; v < c->natv; ++v) ss += c->hsums[6 * v];
+  for (int v = 0; v < c->natv; ++v) ss += c->hsums[W * v];
+  if (andm) {  // weighted dots of the new residual with itself and the history
+    const int m1 = c->opts.anderson_m + 1;
+    auto wdot = [&](int k) {
+      double a = 0.0;
+      for (int v = 0; v < c->natv; ++v) a += c->hsums[W * v + k] / c->rho[v];
+      return a;
+    };
+    c->D[(size_t)c->cur_rb * m1 + c->cur_rb] = wdot(0);
+    for (size_t p = 0; p < c->hist.size(); ++p) {
+      const int b = c->hist[p].rb;
+      c->D[(size_t)c->cur_rb * m1 + b] = c->D[(size_t)b * m1 + c->cur_rb] = wdot(1 + (int)p);
+    }
+  }
   c->iT_last = c->iT_cur;
-  c->iR_last = c->iR_cur;
+  c->Rlast = Rout;
+  c->fresh = true;
   c->have_step = true;
   if (rms) *rms = std::sqrt(ss / ((double)c->N * c->natv));  // cal_rms.cu:24
   return RISM_OK;
@@ -567,8 +740,10 @@ int rism_calculate(rism_ctx *c, double *rms) {
 
 int rism_anderson(rism_ctx *c) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
-  if (!c->have_step || c->iR_last != c->iR_cur)
+  if (!c->have_step || !c->fresh)
     return rism::set_error(RISM_ERR_STATE, "rism_anderson needs a preceding rism_calculate");
+  c->fresh = false;
+  if (c->opts.anderson_m > 0) return anderson_m_step(c);
   const size_t nn = c->N * c->natv;
   const int b = c->binary;
   const int oldR = c->iR_slot[b];
@@ -662,7 +837,7 @@ int rism_thermo(rism_ctx *c, rism_thermo_out *out) {
   const int tb = rism::thermo_blocks(c->N);
   {
     KTime kt(c, RISM_K_THERMO);
-    HIPCHECK(rism::run_thermo_partial(c->T[c->iT_cur], c->T[c->iT_last], c->u, c->R[c->iR_last],
+    HIPCHECK(rism::run_thermo_partial(c->T[c->iT_cur], c->T[c->iT_last], c->u, c->Rlast,
                                       c->P, c->N, c->tpartial, c->st));
   }
   for (int v = 0; v < c->natv; ++v)
@@ -721,7 +896,7 @@ int rism_get_field(rism_ctx *c, int which, int site, double *host) {
       break;
     case RISM_FIELD_R:
       if (!c->have_step) return rism::set_error(RISM_ERR_STATE, "no step taken yet");
-      src = c->R[c->iR_last] + off;
+      src = c->Rlast + off;
       break;
     case RISM_FIELD_G:
     case RISM_FIELD_H:
@@ -729,7 +904,7 @@ int rism_get_field(rism_ctx *c, int which, int site, double *host) {
       const int w = which == RISM_FIELD_G ? 1 : (which == RISM_FIELD_H ? 2 : 3);
       if (w != 1 && !c->have_step) return rism::set_error(RISM_ERR_STATE, "no step taken yet");
       const double *t = c->T[c->have_step ? c->iT_last : c->iT_cur] + off;
-      HIPCHECK(rism::run_fields(t, c->u + off, c->R[c->iR_last] + off, c->P, c->N, w, c->scratch, c->st));
+      HIPCHECK(rism::run_fields(t, c->u + off, c->Rlast + off, c->P, c->N, w, c->scratch, c->st));
       src = c->scratch;
       break;
     }

@@ -17,7 +17,6 @@ struct Params {
   int clos;          // 0 KH, 1 HNC
   int mode;          // 0 solver step; 1 plain forward transform; 2 plain inverse
   int shell;         // 1: chi from the cubic-box shell table, 0: spline at |k|
-  int ztile;         // z-pass tile width request (0: default, 8: narrower)
   double dv;         // volume element (forward scale)
   double inv_scale;  // 1 / (dv nx ny (nz/2)): inverse scale of the three passes
   double dr[3];      // grid spacing
@@ -26,6 +25,18 @@ struct Params {
   // chi spline: knots x_i = x0 + i/idk approximately (exact values in `knots`)
   int np;
   double x0, idk;
+};
+
+// History of the Anderson(m) accelerator passed to kernels by value.
+#define RISM_MAX_HIST 16
+struct Hist {
+  const double *r[RISM_MAX_HIST];  // residual history R_p, p < nh
+  int nh;
+};
+struct HistUpdate {
+  const double *p[RISM_MAX_HIST];  // P_p = T_p + beta R_p, p < nh
+  double x[RISM_MAX_HIST];         // mixing coefficients
+  int nh;
 };
 
 bool fft_length_supported(int L);
@@ -47,6 +58,13 @@ hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const do
                     double *R, const double *R0, const double *R1, double *partial,
                     const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,
                     hipStream_t st);
+// Anderson(m) variant: partial sums R.R and R.R_p for the nh history vectors
+// (width 1 + RISM_MAX_HIST per workgroup).
+hipError_t run_zinv_hist(const Params &P, const cplx *spec, const double *t, const double *u,
+                         double *R, const Hist &H, double *partial, const cplx *twM,
+                         const cplx *zpre, const cplx *zpost, hipStream_t st);
+hipError_t run_andm_update(double *tnew, const double *t, const double *R, const HistUpdate &H,
+                           double *Pb, double beta, size_t n, hipStream_t st);
 hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out,
                            hipStream_t st);
 hipError_t run_an2_picard(double *tnew, const double *t, const double *R, double *Pb, double m,

@@ -71,6 +71,32 @@ __global__ void k_an2_update(double2 *__restrict__ tnew, const double2 *__restri
 }
 
 // ---------------------------------------------------------------------------
+// Anderson(m): the AN2 update generalised to nh history slots,
+//   t' = Q + sum_p x_p (P_p - Q),  Q = t + beta R,
+// with the retired slot's P overwritten by Q.  Pb may alias one of the P_p;
+// each element is read before the same thread writes it.
+// ---------------------------------------------------------------------------
+__global__ void k_andm_update(double2 *__restrict__ tnew, const double2 *__restrict__ t,
+                              const double2 *__restrict__ R, HistUpdate H, double2 *Pb,
+                              double beta, size_t n2) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n2) return;
+  const double2 ti = t[i], ri = R[i];
+  const double qx = ti.x + beta * ri.x, qy = ti.y + beta * ri.y;
+  double ax = qx, ay = qy;
+#pragma unroll
+  for (int q = 0; q < RISM_MAX_HIST; ++q) {
+    if (q < H.nh) {
+      const double2 p = reinterpret_cast<const double2 *>(H.p[q])[i];
+      ax = fma(H.x[q], p.x - qx, ax);
+      ay = fma(H.x[q], p.y - qy, ay);
+    }
+  }
+  tnew[i] = make_double2(ax, ay);
+  Pb[i] = make_double2(qx, qy);
+}
+
+// ---------------------------------------------------------------------------
 // set-up: potentials (cal_LJ.cu:56-81, cal_Coulomb.cu:40-62, cal_potential.cu:39-43)
 // ---------------------------------------------------------------------------
 // Each thread owns PX consecutive x points of one (y, z) line and a group of
@@ -359,6 +385,15 @@ hipError_t run_an2_update(double *tnew, const double *t, const double *R, const 
   hipLaunchKernelGGL(k_an2_update, dim3(nblk(n2, 256)), dim3(256), 0, st, (double2 *)tnew,
                      (const double2 *)t, (const double2 *)R, (const double2 *)P0,
                      (const double2 *)P1, (double2 *)Pb, s1, s2, m, n2);
+  return hipGetLastError();
+}
+
+hipError_t run_andm_update(double *tnew, const double *t, const double *R, const HistUpdate &H,
+                           double *Pb, double beta, size_t n, hipStream_t st) {
+  if ((n & 1) || H.nh < 0 || H.nh > RISM_MAX_HIST) return hipErrorInvalidValue;
+  const size_t n2 = n / 2;
+  hipLaunchKernelGGL(k_andm_update, dim3(nblk(n2, 256)), dim3(256), 0, st, (double2 *)tnew,
+                     (const double2 *)t, (const double2 *)R, H, (double2 *)Pb, beta, n2);
   return hipGetLastError();
 }
 

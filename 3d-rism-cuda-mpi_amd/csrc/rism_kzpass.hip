@@ -71,18 +71,29 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
 // ---------------------------------------------------------------------------
 // 5. half-complex->real inverse DFT along z, residual, FP64 partial sums
 // ---------------------------------------------------------------------------
-// Partial sums per workgroup (6 doubles): R.R, R.(R-R0), R.(R-R1),
-// (R-R0)^2, (R-R0)(R-R1), (R-R1)^2 (anderson_cuda.cu:34-40, cal_rms.cu:20).
-template <int M, int B>
+// Partial sums per workgroup, HM = 0 (AN2, 6 doubles): R.R, R.(R-R0),
+// R.(R-R1), (R-R0)^2, (R-R0)(R-R1), (R-R1)^2 (anderson_cuda.cu:34-40,
+// cal_rms.cu:20); HM = 1 (Anderson(m), 1 + RISM_MAX_HIST doubles): R.R and
+// R.R_p for the nh history residuals.
+template <int HM>
+struct ZinvSums {
+  static constexpr int W = HM == 0 ? 6 : 1 + RISM_MAX_HIST;
+};
+
+template <int M, int B, int HM>
 __global__ void __launch_bounds__(B *FftShape<M>::T)
     k_zinv_resid(const cplx *__restrict__ spec, const double *__restrict__ t,
                  const double *__restrict__ u, double *__restrict__ R,
-                 const double *__restrict__ R0, const double *__restrict__ R1, Params P,
-                 double *__restrict__ partial, const cplx *__restrict__ twM,
+                 const double *__restrict__ R0, const double *__restrict__ R1, Hist H,
+                 Params P, double *__restrict__ partial, const cplx *__restrict__ twM,
                  const cplx *__restrict__ zpre, const cplx *__restrict__ zpost, int use_hist) {
   constexpr int T = FftShape<M>::T, CS = FftShape<M>::CS;
   constexpr int NT = B * T;
-  __shared__ cplx lds[B * CS];
+  constexpr int W = ZinvSums<HM>::W;
+  // the LDS tile doubles as the reduction buffer (NT * W doubles when the
+  // workgroup is not made of whole waves)
+  constexpr int LDSN = (B * CS > (NT * W + 1) / 2) ? B * CS : (NT * W + 1) / 2;
+  __shared__ cplx lds[LDSN];
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   const int ntx = (P.nx + B - 1) / B;
@@ -119,8 +130,11 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   const size_t base = (size_t)y * P.nx + (ok ? x : 0);
   const int clos = P.clos;
   const double scale = P.inv_scale;
-  double s[6] = {0, 0, 0, 0, 0, 0};
+  double s[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) s[k] = 0.0;
   const int plain = P.mode != 0;
+  const size_t voff = (size_t)v * N;
   auto resid = [&](int z, double h) {
     const size_t i = base + (size_t)z * plane;
     if (plain) {  // plain inverse transform: write h itself
@@ -131,13 +145,19 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
     const double r = h + 1.0 - g;  // tr (calculate_cuda.cu:58)
     Rv[i] = r;
     s[0] += r * r;
-    if (use_hist) {
-      const double d0 = r - R0v[i], d1 = r - R1v[i];
-      s[1] += r * d0;
-      s[2] += r * d1;
-      s[3] += d0 * d0;
-      s[4] += d0 * d1;
-      s[5] += d1 * d1;
+    if constexpr (HM == 0) {
+      if (use_hist) {
+        const double d0 = r - R0v[i], d1 = r - R1v[i];
+        s[1] += r * d0;
+        s[2] += r * d1;
+        s[3] += d0 * d0;
+        s[4] += d0 * d1;
+        s[5] += d1 * d1;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < RISM_MAX_HIST; ++q)
+        if (q < H.nh) s[1 + q] += r * H.r[q][voff + i];
     }
   };
   auto load = [&](int) -> cplx { return mk(0.0, 0.0); };
@@ -149,14 +169,14 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   };
   fft_column<M, +1, false, true>(col, j, twM, load, store);
 
-  // workgroup reduction of the six sums: wave shuffles then LDS when the
+  // workgroup reduction of the W sums: wave shuffles then LDS when the
   // workgroup is made of whole waves, LDS only otherwise (tiny grids)
   __syncthreads();
   double *red = reinterpret_cast<double *>(lds);
   int nparts;
   if constexpr (NT % 64 == 0) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < W; ++k) {
       double a = s[k];
       for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
       s[k] = a;
@@ -164,33 +184,23 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
     const int lane = tid & 63, wv = tid >> 6;
     if (lane == 0) {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) red[wv * 6 + k] = s[k];
+      for (int k = 0; k < W; ++k) red[wv * W + k] = s[k];
     }
     nparts = NT / 64;
   } else {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) red[tid * 6 + k] = s[k];
+    for (int k = 0; k < W; ++k) red[tid * W + k] = s[k];
     nparts = NT;
   }
   __syncthreads();
-  if (tid < 6) {
+  if (tid < W) {
     double a = 0.0;
-    for (int w = 0; w < nparts; ++w) a += red[w * 6 + tid];
+    for (int w = 0; w < nparts; ++w) a += red[w * W + tid];
     const size_t blk = (size_t)v * ntx * P.ny + (size_t)y * ntx + tx;  // [site][tile]
-    partial[blk * 6 + tid] = a;
+    partial[blk * W + tid] = a;
   }
 }
 
-// Tile width of the z passes: the default of tile_cols<M>(), or half of it
-// when P.ztile asks for narrower tiles (more workgroups per CU).
-template <int M>
-constexpr int ztile_narrow() {
-  return tile_cols<M>() >= 8 ? tile_cols<M>() / 2 : tile_cols<M>();
-}
-template <int M>
-static int ztile_of(const Params &P) {
-  return (P.ztile > 0 && P.ztile < tile_cols<M>()) ? ztile_narrow<M>() : tile_cols<M>();
-}
 static inline unsigned zgrid(const Params &P, int B) {
   return (unsigned)(((P.nx + B - 1) / B) * P.ny * P.natv);
 }
@@ -198,26 +208,18 @@ template <int M>
 static void launch_zfwd(const Params &P, const double *t, const double *u, const double *fr,
                         cplx *spec, const cplx *twM, const cplx *zpre, const cplx *zpost,
                         hipStream_t st) {
-  constexpr int B = tile_cols<M>(), B2 = ztile_narrow<M>();
-  if (ztile_of<M>(P) == B)
-    hipLaunchKernelGGL((k_zfwd_closure<M, B>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0,
-                       st, t, u, fr, P, spec, twM, zpre, zpost);
-  else
-    hipLaunchKernelGGL((k_zfwd_closure<M, B2>), dim3(zgrid(P, B2)), dim3(B2 * FftShape<M>::T),
-                       0, st, t, u, fr, P, spec, twM, zpre, zpost);
+  constexpr int B = tile_cols<M>();
+  hipLaunchKernelGGL((k_zfwd_closure<M, B>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0, st,
+                     t, u, fr, P, spec, twM, zpre, zpost);
 }
-template <int M>
+template <int M, int HM>
 static void launch_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
-                        double *R, const double *R0, const double *R1, double *partial,
-                        const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,
-                        hipStream_t st) {
-  constexpr int B = tile_cols<M>(), B2 = ztile_narrow<M>();
-  if (ztile_of<M>(P) == B)
-    hipLaunchKernelGGL((k_zinv_resid<M, B>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0, st,
-                       spec, t, u, R, R0, R1, P, partial, twM, zpre, zpost, use_hist);
-  else
-    hipLaunchKernelGGL((k_zinv_resid<M, B2>), dim3(zgrid(P, B2)), dim3(B2 * FftShape<M>::T), 0,
-                       st, spec, t, u, R, R0, R1, P, partial, twM, zpre, zpost, use_hist);
+                        double *R, const double *R0, const double *R1, const Hist &H,
+                        double *partial, const cplx *twM, const cplx *zpre, const cplx *zpost,
+                        int use_hist, hipStream_t st) {
+  constexpr int B = tile_cols<M>();
+  hipLaunchKernelGGL((k_zinv_resid<M, B, HM>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0,
+                     st, spec, t, u, R, R0, R1, H, P, partial, twM, zpre, zpost, use_hist);
 }
 bool fft_length_supported(int L) {
   switch (L) {
@@ -235,7 +237,7 @@ size_t zinv_num_blocks(const Params &P) {
   switch (M) {
 #define X(n) \
   case n:    \
-    B = ztile_of<n>(P); \
+    B = tile_cols<n>(); \
     break;
     RISM_LENGTHS(X)
 #undef X
@@ -266,7 +268,23 @@ hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const do
   switch (P.nz / 2) {
 #define X(n)                                                                             \
   case n:                                                                                \
-    launch_zinv<n>(P, spec, t, u, R, R0, R1, partial, twM, zpre, zpost, use_hist, st); \
+    launch_zinv<n, 0>(P, spec, t, u, R, R0, R1, Hist{}, partial, twM, zpre, zpost, use_hist, st); \
+    break;
+    RISM_LENGTHS(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t run_zinv_hist(const Params &P, const cplx *spec, const double *t, const double *u,
+                         double *R, const Hist &H, double *partial, const cplx *twM,
+                         const cplx *zpre, const cplx *zpost, hipStream_t st) {
+  switch (P.nz / 2) {
+#define X(n)                                                                                    \
+  case n:                                                                                       \
+    launch_zinv<n, 1>(P, spec, t, u, R, nullptr, nullptr, H, partial, twM, zpre, zpost, 0, st); \
     break;
     RISM_LENGTHS(X)
 #undef X

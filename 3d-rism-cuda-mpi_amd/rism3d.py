@@ -68,7 +68,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("closure", ctypes.c_int), ("an2_count", ctypes.c_int),
                 ("an2_m", ctypes.c_double), ("an2_mp", ctypes.c_double),
                 ("ad_switch", ctypes.c_int), ("lambda_", ctypes.c_double),
-                ("verbose", ctypes.c_int), ("chi_spline", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("chi_spline", ctypes.c_int),
+                ("anderson_m", ctypes.c_int)]
 
 
 class Thermo(ctypes.Structure):
@@ -266,7 +267,7 @@ class RISM3D:
     """
 
     def __init__(self, grid, box, solvent, solute, closure=KH, count=2, m=0.6, mp=0.2,
-                 ad_switch=0, lam=1.0, verbose=False, chi_spline=False):
+                 ad_switch=0, lam=1.0, verbose=False, chi_spline=False, anderson_m=0):
         self.grid = [int(g) for g in grid]
         self.box = [float(b) for b in box]
         self.natv = int(solvent["natv"])
@@ -285,7 +286,7 @@ class RISM3D:
         su = SoluteC(natu, *[_ptr(a) for a in us])
         g = Grid(self.grid[0], self.grid[1], self.grid[2], (ctypes.c_double * 3)(*self.box))
         op = Opts(int(closure), int(count), float(m), float(mp), int(ad_switch), float(lam),
-                  int(bool(verbose)), int(bool(chi_spline)))
+                  int(bool(verbose)), int(bool(chi_spline)), int(anderson_m))
         err = ctypes.c_int(0)
         self._ctx = lib().rism_create(ctypes.byref(g), ctypes.byref(sv), ctypes.byref(su),
                                       ctypes.byref(op), ctypes.byref(err))
@@ -295,10 +296,10 @@ class RISM3D:
 
     @classmethod
     def from_inp(cls, control, structure=None, center=True, q0=False, solvent_dir=None,
-                 ad_switch=0, lam=1.0, verbose=False, chi_spline=False):
+                 ad_switch=0, lam=1.0, verbose=False, chi_spline=False, anderson_m=0):
         d = read_input(control, structure, center, q0, solvent_dir)
         obj = cls(d["grid"], d["box"], d, d, d["closure"], d["count"], d["m"], d["mp"],
-                  ad_switch, lam, verbose, chi_spline)
+                  ad_switch, lam, verbose, chi_spline, anderson_m)
         obj.input = d
         obj.shift = d["shift"]
         return obj

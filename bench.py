@@ -84,13 +84,15 @@ MEASURED_TRAFFIC_CFG2 = {
 }
 
 
-def algorithmic_bytes(natv, N, hist=True):
-    """Minimal HBM bytes per launch of each hot-loop kernel (FP64 words of 8 B):
+def algorithmic_bytes(natv, N, hist=2):
+    """Minimal HBM bytes per launch of each hot-loop kernel (FP64 words of 8 B)
+    with `hist` history residuals (2 for AN2, m for Anderson(m)):
     zfwd reads t, u (2n) and fr (1), writes the half spectrum (n);
     each y pass reads and writes the spectrum (2n);
     xpass_oz reads the spectrum (n) and the half fk (1), writes (n);
-    zinv reads the spectrum, t, u (3n) and R0, R1 (2n), writes R (n);
-    an2_update reads t, R, P0, P1 (4n), writes t', P_b (2n)."""
+    zinv reads the spectrum, t, u (3n) and the history residuals (hist n),
+    writes R (n);
+    the update reads t, R and the history P_p (hist n), writes t', P_new (2n)."""
     w = 8.0 * N
     n = natv
     return {
@@ -98,8 +100,8 @@ def algorithmic_bytes(natv, N, hist=True):
         "ypass_fwd": 2 * n * w,
         "xpass_oz": (2 * n + 1) * w,
         "ypass_inv": 2 * n * w,
-        "zinv_resid": (6 * n if hist else 4 * n) * w,
-        "an2_update": 6 * n * w,
+        "zinv_resid": (4 + hist) * n * w,
+        "an2_update": (4 + hist) * n * w,
     }
 
 
@@ -162,6 +164,10 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--anderson-m", type=int, default=None,
+                    help="history length of Anderson(m) mixing (0: the reference's AN2)")
+    ap.add_argument("--beta", type=float, default=0.6,
+                    help="mixing coefficient (AN2's m; beta of Anderson(m))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--maxstep", type=int, default=10000,
                     help="iteration cap (profiling runs only; the metric needs convergence)")
@@ -191,7 +197,8 @@ def main():
     solu = {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"], "uxyz": sol["xyz"]}
     log(f"[bench] workload {args.workload}: grid {grid}, natu {len(sol['sig'])}, "
         f"natv {solv['natv']}")
-    g = rism.RISM3D(grid, box, solv, solu, rism.KH, 2, 0.6, 0.2)
+    am = args.anderson_m if args.anderson_m is not None else 0
+    g = rism.RISM3D(grid, box, solv, solu, rism.KH, 2, args.beta, 0.2, anderson_m=am)
     for i in range(args.warmup):
         th, tm = g.run(maxstep=args.maxstep, tol=1e-6)
         log(f"[bench] warmup {i}: {tm}")
@@ -216,7 +223,7 @@ def main():
     iters = tm["steps"]
 
     # roofline of the dominant hot-loop kernel
-    ab = algorithmic_bytes(natv, N)
+    ab = algorithmic_bytes(natv, N, am if am > 0 else 2)
     hot = {k: v for k, v in ks.items() if k in ab and v[0] > 0}
     dom = max(hot, key=lambda k: hot[k][1])
     launches, total_ms = hot[dom]
@@ -251,7 +258,7 @@ def main():
         "config": {
             "workload": desc,
             "grid": list(grid), "box": list(box), "natu": int(len(sol["sig"])),
-            "natv": int(natv), "iterations": iters, "status": tm["status"],
+            "natv": int(natv), "anderson_m": am, "iterations": iters, "status": tm["status"],
             "setup_ms": tm["setup_ms"], "iterate_ms": tm["iterate_ms"],
             "thermo_ms": tm["thermo_ms"], "ms_per_iteration": iter_ms,
             "iteration_GBps_algorithmic": iter_bytes / (iter_ms / 1000.0) / 1e9,
@@ -265,7 +272,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC_CFG2.get(dom) if args.workload == "cfg2" else None,
+            "traffic": (MEASURED_TRAFFIC_CFG2.get(dom)
+                        if args.workload == "cfg2" and am == 0 else None),
             "algorithmic_bytes_per_launch": ab[dom],
             "avg_launch_ms": avg_s * 1000.0,
         },

@@ -97,6 +97,10 @@ typedef struct {
   int verbose;           /* print the reference's progress lines            */
   int chi_spline;        /* 1: evaluate chi_vv(|k|) from the spline at every  */
                          /* k; 0: per-shell table when the box is cubic       */
+  int anderson_m;        /* 0: the reference's two-slot AN2 (anderson.cu);    */
+                         /* 1..16: Anderson mixing over that many history    */
+                         /* pairs, beta = an2_m, after an2_count Picard      */
+                         /* steps (no counterpart in the reference)          */
 } rism_opts;
 
 /* Thermodynamics of the converged state (output.cu:23-46, output_xmu.cu). */

@@ -17,6 +17,13 @@ Reference files restated here:
   * loop                     -- iterate.cu:48-82
   * thermodynamics, .xmu     -- cal_exchem.cu, cal_pmv.cu, cal_pressure.cu,
                                 output_xmu.cu:7-82
+
+``AndersonM`` is the one piece with no counterpart in the reference: Anderson
+mixing over m history pairs, the generalisation of AN2 (anderson.cu) that the
+product offers as ``rism_opts.anderson_m``.  It is written from its definition
+(a weighted least-squares problem solved with ``numpy.linalg.lstsq`` on the
+explicit difference vectors) and pinned by the finite termination of Anderson
+mixing on a linear map (tests/test_oracle_iter.py).
 """
 from __future__ import annotations
 
@@ -372,11 +379,77 @@ class OracleRISM:
         self.result = Result(steps, st.value, hist[:steps].copy(), t, g, h, R)
         return self.result
 
+    def solve_anderson_m(self, m, beta=None, count=None, t=None, maxstep=None):
+        """The same loop (iterate.cu:48-72) with AndersonM in place of AN2."""
+        n = self.solv.natv
+        t = (self.t0 if t is None else t).copy()
+        acc = AndersonM(n, self.N, self.solv.rho, m, self.m if beta is None else beta,
+                        self.count if count is None else count)
+        ms = self.maxstep if maxstep is None else int(maxstep)
+        hist, status = [], 2
+        g = h = R = None
+        for _ in range(ms):
+            rms, g, R, h = self.calculate(t)
+            hist.append(rms)
+            if not np.isfinite(rms):
+                status = 1
+                break
+            if rms <= self.tol:
+                status = 0
+                break
+            t = acc.step(t, R)
+        self.result = Result(len(hist), status, np.array(hist), t, g, h, R)
+        self.accel = acc
+        return self.result
+
     def thermo(self, h=None, t=None):
         r = self.result if h is None else None
         h = r.h if h is None else h
         t = r.t if t is None else t
         return thermo(self.solv, self.dv, self.clos, h, t)
+
+
+class AndersonM:
+    """Anderson mixing over m history pairs (no counterpart in the reference).
+
+    With Q = t + beta R and P_p = t_p + beta R_p of the history pairs, a step
+    returns  t' = Q + sum_p th_p (P_p - Q),  where th minimises the weighted
+    norm |R - sum_p th_p (R - R_p)|, the weight of site v being 1/rho_v as in
+    AN2's cal_theta (anderson.cu:81-102).  The first ``count`` steps are the
+    reference's Picard steps t' = t + R (anderson.cu:54-60).  Every step then
+    stores (R, Q) and drops the oldest pair beyond m.
+    """
+
+    def __init__(self, natv, N, rho, m, beta, count=0):
+        self.m, self.beta, self.count = int(m), float(beta), int(count)
+        self.w = np.repeat(1.0 / np.asarray(rho, dtype=float), N)
+        self.R, self.P = [], []
+        self.theta = np.zeros(0)
+
+    def coefficients(self, R):
+        if not self.R:
+            return np.zeros(0)
+        sw = np.sqrt(self.w)
+        M = np.stack([(R - Rp) * sw for Rp in self.R], axis=1)
+        return np.linalg.lstsq(M, R * sw, rcond=None)[0]
+
+    def step(self, t, R):
+        Q = t + self.beta * R
+        if self.count > 0:
+            self.count -= 1
+            tn = t + R
+            self.theta = np.zeros(0)
+        else:
+            self.theta = self.coefficients(R)
+            tn = Q.copy()
+            for th, Pp in zip(self.theta, self.P):
+                tn += th * (Pp - Q)
+        self.R.append(R.copy())
+        self.P.append(Q)
+        if len(self.R) > self.m:
+            self.R.pop(0)
+            self.P.pop(0)
+        return tn
 
 
 def thermo(solv: Solvent, dv, clos, h, t):

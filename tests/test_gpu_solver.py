@@ -171,3 +171,47 @@ def test_krange_error(rism, oracle, gpu, golden):
     with pytest.raises(rism.RismError) as e:
         rism.RISM3D((64, 64, 64), (2.0, 2.0, 2.0), d, sol)
     assert e.value.code == rism.RISM_ERR_KRANGE
+
+
+def test_anderson_m_trajectory(rism, oracle, gpu, case24):
+    """Anderson(m) (rism_opts.anderson_m, no counterpart in the reference):
+    t after 12 steps (2 Picard + 10 mixing steps, history filling up to m = 5
+    and then rotating) follows the oracle's AndersonM, whose least squares is
+    solved from the explicit difference vectors."""
+    sol, s, d, grid, box = case24
+    o = oracle.OracleRISM(grid, box, s, sol, "KH", count=2, m=0.6, mp=0.2, tol=1e-12).setup()
+    ro = o.solve_anderson_m(5, maxstep=12)
+    g = rism.RISM3D(grid, box, d, _solute(sol), count=2, m=0.6, mp=0.2, anderson_m=5)
+    g.initialize()
+    st, n, rms = g.iterate(maxstep=12, tol=1e-12)
+    assert (st, n) == (rism.STATUS_MAXSTEP, 12)
+    assert abs(rms - ro.rms[-1]) < 1e-9 * ro.rms[-1]
+    t = g.fields(rism.FIELD_T).reshape(-1)
+    assert np.abs(t - ro.t).max() < 1e-9 * np.abs(ro.t).max()
+    g.close()
+
+
+def test_anderson_m_same_fixed_point_as_an2(rism, oracle, gpu, case24):
+    """Anderson(10) and the reference's AN2 reach the same solution: solved
+    to rms 1e-9, the .xmu quantities agree to 1e-6 relative."""
+    sol, s, d, grid, box = case24
+    out = {}
+    for am in (0, 10):
+        g = rism.RISM3D(grid, box, d, _solute(sol), anderson_m=am)
+        g.initialize()
+        st, n, rms = g.iterate(maxstep=2000, tol=1e-9)
+        assert st == rism.STATUS_CONVERGED, (am, n, rms)
+        out[am] = (n, g.thermo())
+        g.close()
+    print("steps AN2", out[0][0], "Anderson(10)", out[10][0])
+    for k in ("SFE_SC", "SFE_GF", "PMV"):
+        a, b = out[0][1][k], out[10][1][k]
+        assert abs(a - b) <= 1e-6 * max(1.0, abs(a)), (k, a, b)
+    assert out[10][0] < out[0][0]
+
+
+def test_anderson_m_rejects_bad_history(rism, oracle, gpu, case24):
+    sol, s, d, grid, box = case24
+    with pytest.raises(rism.RismError) as e:
+        rism.RISM3D(grid, box, d, _solute(sol), anderson_m=17)
+    assert e.value.code == rism.RISM_ERR_ARG

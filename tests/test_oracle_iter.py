@@ -179,3 +179,87 @@ def test_oracle_mobley_rerun(golden, oracle):
     # agree to rounding propagated through ~100 steps, not bit for bit
     assert abs(o.result.steps - rec["steps"]) <= 1
     assert th["SFE_SC"] == pytest.approx(rec["SFE_SC"], rel=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# Anderson(m): no counterpart in the reference; pinned by what the
+# mathematics fixes.
+# ---------------------------------------------------------------------------
+def _linear(seed, natv, N, scale=0.3):
+    rng = np.random.default_rng(seed)
+    n = natv * N
+    A = scale * rng.normal(size=(n, n)) / np.sqrt(n) * 2.0
+    b = rng.normal(size=n)
+    return A, b
+
+
+@pytest.mark.parametrize("beta", [1.0, 0.6, 0.3])
+def test_anderson_m_finite_termination(oracle, beta):
+    """Anderson mixing with a history at least as long as the dimension d
+    solves a linear fixed point t = A t + b exactly after d + 1 steps (it is
+    GMRES in disguise); a wrong sign, a dropped P_p term or a wrong weight
+    breaks this."""
+    natv, N = 2, 3
+    d = natv * N
+    A, b = _linear(11, natv, N)
+    rho = np.array([0.02, 0.05])
+    acc = oracle.AndersonM(natv, N, rho, m=10, beta=beta, count=0)
+    t = np.zeros(d)
+    res = []
+    for _ in range(d + 2):
+        R = A @ t + b - t
+        res.append(np.linalg.norm(R))
+        t = acc.step(t, R)
+    assert res[d + 1] < 1e-11 * res[0]
+    assert min(res[:d]) > 1e-6 * res[0]  # not trivially converged earlier
+    exact = np.linalg.solve(np.eye(d) - A, b)
+    assert np.abs(t - exact).max() < 1e-10 * np.abs(exact).max()
+
+
+def test_anderson_m_warmup_and_first_step(oracle):
+    """count > 0: the reference's Picard step t + R; count = 0 and an empty
+    history: t + beta R; one history pair: the 1-D least squares closed form
+    th = <R, R - R0>_w / |R - R0|_w^2."""
+    natv, N = 2, 4
+    A, b = _linear(12, natv, N)
+    rho = np.array([0.03, 0.01])
+    w = np.repeat(1.0 / rho, N)
+    beta = 0.6
+    acc = oracle.AndersonM(natv, N, rho, m=3, beta=beta, count=1)
+    t0 = np.linspace(-1, 1, natv * N)
+    R0 = A @ t0 + b - t0
+    t1 = acc.step(t0, R0)
+    assert np.array_equal(t1, t0 + R0)
+    R1 = A @ t1 + b - t1
+    t2 = acc.step(t1, R1)
+    D = R1 - R0
+    th = (w @ (R1 * D)) / (w @ (D * D))
+    Q = t1 + beta * R1
+    want = Q + th * ((t0 + beta * R0) - Q)
+    assert np.allclose(t2, want, rtol=1e-13, atol=1e-14)
+    acc0 = oracle.AndersonM(natv, N, rho, m=3, beta=beta, count=0)
+    assert np.array_equal(acc0.step(t0, R0), t0 + beta * R0)
+
+
+def test_anderson_m_two_matches_an2_least_squares(oracle):
+    """With m = 2 the least squares is AN2's cal_theta (anderson.cu:81-102):
+    the coefficients equal the reference leg() solution x of the 2x2
+    weighted normal equations."""
+    natv, N = 2, 16
+    A, b = _linear(13, natv, N)
+    rho = np.array([0.02, 0.05])
+    acc = oracle.AndersonM(natv, N, rho, m=2, beta=0.6, count=2)
+    w = np.repeat(1.0 / rho, N)
+    t = np.zeros(natv * N)
+    Rs = []
+    for _ in range(2):
+        R = A @ t + b - t
+        Rs.append(R)
+        t = acc.step(t, R)
+    R2 = A @ t + b - t
+    D0, D1 = R2 - Rs[0], R2 - Rs[1]
+    G = np.array([w @ (D0 * D0), w @ (D0 * D1), w @ (D0 * D1), w @ (D1 * D1)])
+    c = np.array([w @ (R2 * D0), w @ (R2 * D1)])
+    x = np.zeros(2)
+    oracle.lib().or_leg2(G, c, x)
+    assert np.allclose(acc.coefficients(R2), x, rtol=1e-10, atol=1e-13)
