@@ -10,10 +10,29 @@
 
 namespace rism {
 
-__device__ __forceinline__ double closure_g(int clos, double t, double u) {
-  const double e = -u + t;  // calculate_cuda.cu:4
-  if (clos == 0 && e >= 0.0) return 1.0 + e;
-  return exp(e);
+__device__ __forceinline__ double rexp(double a) { return exp(a); }
+__device__ __forceinline__ float rexp(float a) { return expf(a); }
+
+// g from t and u in the bulk type: clos 0 KH, 1 HNC (kh / hnc,
+// calculate_cuda.cu:1-16), 10 + n PSE-n (Kast-Kloss partial series: exp(e)
+// for e = -u + t < 0, else sum_{i <= n} e^i / i!; PSE-1 is KH)
+template <class Re>
+__device__ __forceinline__ Re closure_g(int clos, Re t, Re u) {
+  const Re e = -u + t;  // calculate_cuda.cu:4
+  if (clos == 1 || e < (Re)0) return rexp(e);
+  if (clos == 0) return (Re)1 + e;
+  Re s = (Re)1;
+  for (int k = clos - 10; k >= 1; --k) s = (Re)1 + e * s / (Re)k;
+  return s;
+}
+
+// the PSE-n correction to the HNC integrand of the excess chemical
+// potential: e^(n+1) / (n+1)! where e = -u + t > 0
+__device__ __forceinline__ double pse_excess(int n, double e) {
+  if (e <= 0.0) return 0.0;
+  double p = 1.0;
+  for (int k = 1; k <= n + 1; ++k) p *= e / (double)k;
+  return p;
 }
 
 __device__ __forceinline__ int spline_interval(const double *__restrict__ knots, int np,
@@ -32,10 +51,14 @@ __device__ __forceinline__ int spline_interval(const double *__restrict__ knots,
   X(8) X(12) X(16) X(20) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(64) X(72) X(80) X(90) \
   X(96) X(128) X(144) X(160) X(180) X(192) X(256) X(512) X(1024)
 
-template <int L>
+// Columns per workgroup of the strided (y, z) passes: one workgroup's LDS
+// tile stays at <= ~74 KB, and a tile row is 128 B of contiguous x, in both
+// bulk types (FP32 tiles are twice as wide), within 1024 threads.
+template <int L, class Re = double>
 constexpr int tile_cols() {
-  // keep one workgroup's LDS tile at <= ~74 KB of FP64 complex
-  return L >= 1024 ? 4 : (L >= 512 ? 8 : 16);
+  int b = (L >= 1024 ? 4 : (L >= 512 ? 8 : 16)) * (int)(sizeof(double) / sizeof(Re));
+  while (b * FftShape<L>::T > 1024) b /= 2;
+  return b;
 }
 
 static inline unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }

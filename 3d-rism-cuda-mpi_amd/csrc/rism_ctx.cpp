@@ -41,6 +41,8 @@ const int kSumWidth = 1 + RISM_MAX_HIST;
   } while (0)
 
 using rism::cplx;
+using rism::cplxf;
+using rism::cx;
 
 // Natural cubic spline through (xp, yp): the same tridiagonal elimination as
 // spline.cc:4-43 (second derivatives s, then per-interval coefficients
@@ -80,9 +82,23 @@ inline double kcomp(int j, int n, double b) {
   return 2.0 * M_PI * ((double)j - (double)n / 2.0 + 0.5) / b;
 }
 
-struct Tables {
-  cplx *twx = nullptr, *twy = nullptr, *twz = nullptr;
-  cplx *prex = nullptr, *prey = nullptr, *zpre = nullptr, *zpost = nullptr;
+// Twiddle and phase tables of the three axis passes in one bulk type.
+template <class Re>
+struct TablesT {
+  cx<Re> *twx = nullptr, *twy = nullptr, *twz = nullptr;
+  cx<Re> *prex = nullptr, *prey = nullptr, *zpre = nullptr, *zpost = nullptr;
+};
+
+// The device fields of the fixed-point loop in one bulk type: FP64 (the
+// reference's precision) or FP32 (the bulk of the mixed-precision solve,
+// before it hands over to FP64).
+template <class Re>
+struct Level {
+  Re *u = nullptr, *fr = nullptr, *T[2] = {}, *Pslot[2] = {}, *R[3] = {};
+  cx<Re> *fk = nullptr, *spec = nullptr;
+  std::vector<Re *> Rb, Pb;  // Anderson(m) residuals and mixed iterates
+  Re *Rlast = nullptr;       // residual written by the last step at this level
+  TablesT<Re> tb;
 };
 
 std::vector<cplx> roots(int L) {  // e^{-2 pi i q / L}
@@ -136,13 +152,13 @@ struct rism_ctx {
   int natu = 0;
   std::vector<double> atoms_host;  // per atom x y z q sig_uv^2[natv] 4eps_uv[natv]
   rism_opts opts{};
-  // device
-  double *u = nullptr, *fr = nullptr, *T[2] = {}, *Pslot[2] = {}, *R[3] = {};
-  cplx *fk = nullptr, *spec = nullptr;
+  // device: the FP64 level, and the FP32 level of a mixed-precision context
+  Level<double> d;
+  Level<float> f;
+  int prec = 0;  // level the loop runs at: 0 FP64, 1 FP32
   double *partial = nullptr, *tpartial = nullptr, *sums = nullptr, *scratch = nullptr;
   double *atoms = nullptr, *knots = nullptr, *coef = nullptr, *chitab = nullptr;
   size_t zinv_blocks = 0;
-  Tables tb;
   std::vector<void *> allocs;
   // AN2 state (anderson.h:18-40)
   int count = 0, binary = 0;
@@ -151,15 +167,14 @@ struct rism_ctx {
   // Pslot[p] and R_p in R[iR_slot[p]] (rotated by pointer)
   int iT_cur = 0, iR_cur = 0, iR_slot[2] = {1, 2};
   int iT_last = 0;
-  double *Rlast = nullptr;  // residual written by the last rism_calculate
-  bool fresh = false;       // no rism_anderson since that rism_calculate
+  int last_prec = 0;  // level of the last rism_calculate
+  bool fresh = false;  // no rism_anderson since that rism_calculate
   // Anderson(m) state (opts.anderson_m > 0): residuals in Rb[m + 1], mixed
   // iterates P_p = t_p + beta R_p in Pb[m]; hist lists the (R, P) buffer
   // pairs oldest first; D holds the weighted dots <R_a, R_b> by R buffer.
   struct HEnt {
     int rb, pb;
   };
-  std::vector<double *> Rb, Pb;
   std::vector<HEnt> hist;
   int cur_rb = 0, next_pb = 0;
   std::vector<double> D;
@@ -199,6 +214,19 @@ struct rism_ctx {
 };
 
 namespace {
+template <class Re>
+Level<Re> &lv(rism_ctx *c);
+template <>
+Level<double> &lv<double>(rism_ctx *c) {
+  return c->d;
+}
+template <>
+Level<float> &lv<float>(rism_ctx *c) {
+  return c->f;
+}
+}  // namespace
+
+namespace {
 
 template <class Tp>
 int upload(rism_ctx *c, Tp **dst, const std::vector<Tp> &src) {
@@ -208,14 +236,23 @@ int upload(rism_ctx *c, Tp **dst, const std::vector<Tp> &src) {
   return RISM_OK;
 }
 
-int make_tables(rism_ctx *c) {
+// tables computed in FP64 and rounded once to the bulk type
+template <class Re>
+std::vector<cx<Re>> round_to(const std::vector<cplx> &w) {
+  std::vector<cx<Re>> o(w.size());
+  for (size_t i = 0; i < w.size(); ++i) o[i] = cx<Re>{(Re)w[i].x, (Re)w[i].y};
+  return o;
+}
+
+template <class Re>
+int make_tables(rism_ctx *c, TablesT<Re> &tb) {
   const int M = c->nz / 2;
   int rc;
-  if ((rc = upload(c, &c->tb.twx, roots(c->nx)))) return rc;
-  if ((rc = upload(c, &c->tb.twy, roots(c->ny)))) return rc;
-  if ((rc = upload(c, &c->tb.twz, roots(M)))) return rc;
-  if ((rc = upload(c, &c->tb.prex, axis_pre(c->nx)))) return rc;
-  if ((rc = upload(c, &c->tb.prey, axis_pre(c->ny)))) return rc;
+  if ((rc = upload(c, &tb.twx, round_to<Re>(roots(c->nx))))) return rc;
+  if ((rc = upload(c, &tb.twy, round_to<Re>(roots(c->ny))))) return rc;
+  if ((rc = upload(c, &tb.twz, round_to<Re>(roots(M))))) return rc;
+  if ((rc = upload(c, &tb.prex, round_to<Re>(axis_pre(c->nx))))) return rc;
+  if ((rc = upload(c, &tb.prey, round_to<Re>(axis_pre(c->ny))))) return rc;
   std::vector<cplx> zp(M), zq(M);
   for (int p = 0; p < M; ++p) {
     const double a = -M_PI * (double)p / (double)M;
@@ -223,8 +260,8 @@ int make_tables(rism_ctx *c) {
     const double b = -M_PI * (double)(2 * p + 1) / (double)c->nz;
     zq[p] = cplx{std::cos(b), std::sin(b)};
   }
-  if ((rc = upload(c, &c->tb.zpre, zp))) return rc;
-  if ((rc = upload(c, &c->tb.zpost, zq))) return rc;
+  if ((rc = upload(c, &tb.zpre, round_to<Re>(zp)))) return rc;
+  if ((rc = upload(c, &tb.zpost, round_to<Re>(zq)))) return rc;
   return RISM_OK;
 }
 
@@ -328,7 +365,9 @@ void reset_an2(rism_ctx *c) {
   c->iR_slot[0] = 1;
   c->iR_slot[1] = 2;
   c->iT_last = 0;
-  c->Rlast = nullptr;
+  c->d.Rlast = nullptr;
+  c->f.Rlast = nullptr;
+  c->last_prec = c->prec;
   c->fresh = false;
   c->have_step = false;
   c->hist.clear();
@@ -455,15 +494,17 @@ std::vector<double> anderson_m_coeffs(const std::vector<double> &D, int m1, int 
 //   t' = Q + sum_p th_p (P_p - Q),  Q = t + beta R,  beta = an2_m.
 // Either way the new pair (R, Q) enters the history, retiring the oldest
 // pair once m are held.
+template <class Re>
 int anderson_m_step(rism_ctx *c) {
+  Level<Re> &L = lv<Re>(c);
   const size_t nn = c->N * c->natv;
   const int m = c->opts.anderson_m, m1 = m + 1;
-  double *t = c->T[c->iT_cur], *tnew = c->T[1 - c->iT_cur];
-  double *R = c->Rb[c->cur_rb];
+  Re *t = L.T[c->iT_cur], *tnew = L.T[1 - c->iT_cur];
+  Re *R = L.Rb[c->cur_rb];
   // slot for the new pair: the oldest when full, else a fresh one
   const bool full = (int)c->hist.size() == m;
   const rism_ctx::HEnt old = full ? c->hist.front() : rism_ctx::HEnt{-1, c->next_pb};
-  double *Pdst = c->Pb[old.pb];
+  Re *Pdst = L.Pb[old.pb];
   if (c->count > 0) {
     --c->count;
     KTime kt(c, RISM_K_AN2);
@@ -473,10 +514,10 @@ int anderson_m_step(rism_ctx *c) {
     std::vector<int> rb;
     for (const auto &h : c->hist) rb.push_back(h.rb);
     c->lastth = anderson_m_coeffs(c->D, m1, c->cur_rb, rb);
-    rism::HistUpdate H{};
+    rism::HistUpdateT<Re> H{};
     H.nh = (int)c->hist.size();
     for (int p = 0; p < H.nh; ++p) {
-      H.p[p] = c->Pb[c->hist[p].pb];
+      H.p[p] = L.Pb[c->hist[p].pb];
       H.x[p] = c->lastth[p];
     }
     KTime kt(c, RISM_K_AN2);
@@ -491,11 +532,208 @@ int anderson_m_step(rism_ctx *c) {
   } else {
     std::vector<char> used(m1, 0);
     for (const auto &h : c->hist) used[h.rb] = 1;
-    int f = 0;
-    while (used[f]) ++f;
-    c->cur_rb = f;
+    int fi = 0;
+    while (used[fi]) ++fi;
+    c->cur_rb = fi;
   }
   c->iT_cur = 1 - c->iT_cur;
+  return RISM_OK;
+}
+
+// One AN2::calculate (anderson.cu:43-78) at level Re: Picard warm-up, or
+// cal_theta + leg + the two-slot update, from the sums of the last step.
+template <class Re>
+int an2_step(rism_ctx *c) {
+  Level<Re> &L = lv<Re>(c);
+  const size_t nn = c->N * c->natv;
+  const int b = c->binary;
+  const int oldR = c->iR_slot[b];
+  Re *t = L.T[c->iT_cur], *R = L.R[c->iR_cur], *tnew = L.T[1 - c->iT_cur];
+  if (c->count > 0) {  // anderson.cu:54-60
+    --c->count;
+    KTime kt(c, RISM_K_AN2);
+    HIPCHECK(rism::run_an2_picard(tnew, t, R, L.Pslot[b], c->opts.an2_m, nn, c->st));
+  } else {  // anderson.cu:61-72, cal_theta anderson.cu:81-102
+    double cc[2] = {0, 0}, a[4] = {0, 0, 0, 0};
+    for (int v = 0; v < c->natv; ++v) {
+      const double ir = 1.0 / c->rho[v];
+      const double *sv = &c->hsums[6 * v];
+      cc[0] += sv[1] * ir;
+      cc[1] += sv[2] * ir;
+      a[0] += sv[3] * ir;
+      a[1] += sv[4] * ir;
+      a[3] += sv[5] * ir;
+    }
+    a[2] = a[1];
+    // leg(): Gaussian elimination without pivoting (anderson.cu:105-130)
+    double x[2];
+    {
+      double p = a[0];
+      a[1] /= p;
+      double r = cc[0] /= p;
+      double q = a[2];
+      a[3] -= q * a[1];
+      cc[1] -= q * r;
+      x[1] = cc[1] / a[3];
+      x[0] = cc[0] - a[1] * x[1];
+    }
+    const double s1 = x[0] + c->opts.an2_mp * ((b == 0) ? 0 : 1);
+    const double s2 = x[1] + c->opts.an2_mp * ((b == 0) ? 1 : 0);
+    c->lastx[0] = x[0];
+    c->lastx[1] = x[1];
+    c->lasts[0] = s1;
+    c->lasts[1] = s2;
+    KTime kt(c, RISM_K_AN2);
+    HIPCHECK(rism::run_an2_update(tnew, t, R, L.Pslot[0], L.Pslot[1], L.Pslot[b], s1, s2,
+                                  c->opts.an2_m, nn, c->st));
+  }
+  // slot[b] <- (t, R): P_b was rewritten by the kernel, R_b by pointer
+  c->iR_slot[b] = c->iR_cur;
+  c->iR_cur = oldR;
+  c->iT_cur = 1 - c->iT_cur;
+  c->binary = (b == 0) ? 1 : 0;
+  return RISM_OK;
+}
+
+// One calculate() + cal_rms() at level Re (calculate.cu:4-65, cal_rms.cu:14-26)
+template <class Re>
+int calc_step(rism_ctx *c, double *rms) {
+  Level<Re> &L = lv<Re>(c);
+  rism::Params P = c->P;
+  P.mode = 0;
+  Re *t = L.T[c->iT_cur];
+  {
+    KTime kt(c, RISM_K_ZFWD);
+    HIPCHECK(rism::run_zfwd(P, t, L.u, L.fr, L.spec, L.tb.twz, L.tb.zpre, L.tb.zpost, c->st));
+  }
+  {
+    KTime kt(c, RISM_K_YFWD);
+    HIPCHECK(rism::run_y(P, L.spec, -1, L.tb.twy, L.tb.prey, c->st));
+  }
+  {
+    KTime kt(c, RISM_K_XOZ);
+    HIPCHECK(rism::run_xoz(P, L.spec, L.fk, L.tb.twx, L.tb.prex, c->knots, c->coef, c->chitab,
+                           c->st));
+  }
+  {
+    KTime kt(c, RISM_K_YINV);
+    HIPCHECK(rism::run_y(P, L.spec, +1, L.tb.twy, L.tb.prey, c->st));
+  }
+  const bool andm = c->opts.anderson_m > 0;
+  const int W = andm ? kSumWidth : 6;
+  Re *Rout = andm ? L.Rb[c->cur_rb] : L.R[c->iR_cur];
+  {
+    KTime kt(c, RISM_K_ZINV);
+    if (andm) {
+      rism::HistT<Re> H{};
+      H.nh = (int)c->hist.size();
+      for (int p = 0; p < H.nh; ++p) H.r[p] = L.Rb[c->hist[p].rb];
+      HIPCHECK(rism::run_zinv_hist(P, L.spec, t, L.u, Rout, H, c->partial, L.tb.twz, L.tb.zpre,
+                                   L.tb.zpost, c->st));
+    } else {
+      const int use_hist = (c->count == 0);
+      HIPCHECK(rism::run_zinv(P, L.spec, t, L.u, Rout, L.R[c->iR_slot[0]], L.R[c->iR_slot[1]],
+                              c->partial, L.tb.twz, L.tb.zpre, L.tb.zpost, use_hist, c->st));
+    }
+  }
+  const size_t nb = rism::zinv_num_blocks<Re>(c->P) / c->natv;
+  {
+    KTime kt(c, RISM_K_REDUCE);
+    for (int v = 0; v < c->natv; ++v)
+      HIPCHECK(rism::run_reduce_cols(c->partial + (size_t)v * nb * W, nb, W, c->sums + W * v,
+                                     c->st));
+  }
+  HIPCHECK(hipMemcpyAsync(c->hsums.data(), c->sums, sizeof(double) * W * c->natv,
+                          hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  if (c->ktime) ktime_resolve(c);
+  double ss = 0.0;
+  for (int v = 0; v < c->natv; ++v) ss += c->hsums[W * v];
+  if (andm) {  // weighted dots of the new residual with itself and the history
+    const int m1 = c->opts.anderson_m + 1;
+    auto wdot = [&](int k) {
+      double a = 0.0;
+      for (int v = 0; v < c->natv; ++v) a += c->hsums[W * v + k] / c->rho[v];
+      return a;
+    };
+    c->D[(size_t)c->cur_rb * m1 + c->cur_rb] = wdot(0);
+    for (size_t p = 0; p < c->hist.size(); ++p) {
+      const int bb = c->hist[p].rb;
+      c->D[(size_t)c->cur_rb * m1 + bb] = c->D[(size_t)bb * m1 + c->cur_rb] = wdot(1 + (int)p);
+    }
+  }
+  c->iT_last = c->iT_cur;
+  L.Rlast = Rout;
+  c->last_prec = c->prec;
+  c->fresh = true;
+  c->have_step = true;
+  if (rms) *rms = std::sqrt(ss / ((double)c->N * c->natv));  // cal_rms.cu:24
+  return RISM_OK;
+}
+
+// Moves the loop state (t, the AN2 or Anderson(m) slots, the last residual)
+// from one level to the other, keeping every slot index, so the iteration
+// continues where it stands.
+template <class From, class To>
+int convert_state(rism_ctx *c) {
+  Level<From> &A = lv<From>(c);
+  Level<To> &B = lv<To>(c);
+  const size_t nn = c->N * c->natv;
+  auto cv = [&](From *src, To *dst) -> int {
+    if (src && dst) HIPCHECK(rism::run_convert(src, dst, nn, c->st));
+    return RISM_OK;
+  };
+  int rc;
+  for (int i = 0; i < 2; ++i)
+    if ((rc = cv(A.T[i], B.T[i]))) return rc;
+  for (int i = 0; i < 2; ++i)
+    if ((rc = cv(A.Pslot[i], B.Pslot[i]))) return rc;
+  for (int i = 0; i < 3; ++i)
+    if ((rc = cv(A.R[i], B.R[i]))) return rc;
+  for (size_t i = 0; i < A.Rb.size(); ++i)
+    if ((rc = cv(A.Rb[i], B.Rb[i]))) return rc;
+  for (size_t i = 0; i < A.Pb.size(); ++i)
+    if ((rc = cv(A.Pb[i], B.Pb[i]))) return rc;
+  B.Rlast = nullptr;
+  for (int i = 0; i < 3; ++i)
+    if (A.Rlast && A.Rlast == A.R[i]) B.Rlast = B.R[i];
+  for (size_t i = 0; i < A.Rb.size(); ++i)
+    if (A.Rlast && A.Rlast == A.Rb[i]) B.Rlast = B.Rb[i];
+  HIPCHECK(hipStreamSynchronize(c->st));
+  c->last_prec = c->prec = (sizeof(To) == sizeof(double)) ? 0 : 1;
+  return RISM_OK;
+}
+
+// Device buffers of one level: u, fr, fk, t (ping-pong) and the AN2 or
+// Anderson(m) slots; the FP32 level's spectrum lives in the FP64 one's memory
+// (the two levels never run at the same time).
+template <class Re>
+int alloc_level(rism_ctx *c, Level<Re> &L, cplx *spec_alias) {
+  const size_t nn = c->N * c->natv;
+  int rc;
+  if ((rc = c->alloc(&L.u, nn))) return rc;
+  if ((rc = c->alloc(&L.fr, c->N))) return rc;
+  if ((rc = c->alloc(&L.fk, c->N / 2))) return rc;
+  if (spec_alias)
+    L.spec = reinterpret_cast<cx<Re> *>(spec_alias);
+  else if ((rc = c->alloc(&L.spec, nn / 2)))
+    return rc;
+  for (int i = 0; i < 2; ++i)
+    if ((rc = c->alloc(&L.T[i], nn))) return rc;
+  const int am = c->opts.anderson_m;
+  if (am == 0) {
+    for (int i = 0; i < 2; ++i)
+      if ((rc = c->alloc(&L.Pslot[i], nn))) return rc;
+    for (int i = 0; i < 3; ++i)
+      if ((rc = c->alloc(&L.R[i], nn))) return rc;
+  } else {
+    L.Rb.assign(am + 1, nullptr);
+    L.Pb.assign(am, nullptr);
+    for (auto &b : L.Rb)
+      if ((rc = c->alloc(&b, nn))) return rc;
+    for (auto &b : L.Pb)
+      if ((rc = c->alloc(&b, nn))) return rc;
+  }
   return RISM_OK;
 }
 
@@ -537,7 +775,8 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
     *err = rc;
     return nullptr;
   }
-  if (opts->closure != RISM_CLOSURE_KH && opts->closure != RISM_CLOSURE_HNC) {
+  if (opts->closure != RISM_CLOSURE_KH && opts->closure != RISM_CLOSURE_HNC &&
+      !(opts->closure >= RISM_CLOSURE_PSE(1) && opts->closure <= RISM_CLOSURE_PSE(20))) {
     *err = rism::set_error(RISM_ERR_CLOSURE, "Error: Unexpected closure switch ");
     return nullptr;
   }
@@ -590,32 +829,18 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
     }
   }
   if ((rc = upload(c, &c->atoms, c->atoms_host))) return fail(rc);
-  if ((rc = make_tables(c))) return fail(rc);
+  if ((rc = make_tables(c, c->d.tb))) return fail(rc);
   if ((rc = make_chi(c, solvent))) return fail(rc);
 
   const size_t nn = c->N * c->natv;
-  if ((rc = c->alloc(&c->u, nn))) return fail(rc);
-  if ((rc = c->alloc(&c->fr, c->N))) return fail(rc);
-  if ((rc = c->alloc(&c->fk, c->N / 2))) return fail(rc);
-  if ((rc = c->alloc(&c->spec, nn / 2))) return fail(rc);
-  for (int i = 0; i < 2; ++i)
-    if ((rc = c->alloc(&c->T[i], nn))) return fail(rc);
-  const int am = c->opts.anderson_m;
-  if (am == 0) {
-    for (int i = 0; i < 2; ++i)
-      if ((rc = c->alloc(&c->Pslot[i], nn))) return fail(rc);
-    for (int i = 0; i < 3; ++i)
-      if ((rc = c->alloc(&c->R[i], nn))) return fail(rc);
-  } else {
-    c->Rb.assign(am + 1, nullptr);
-    c->Pb.assign(am, nullptr);
-    for (auto &b : c->Rb)
-      if ((rc = c->alloc(&b, nn))) return fail(rc);
-    for (auto &b : c->Pb)
-      if ((rc = c->alloc(&b, nn))) return fail(rc);
-    c->D.assign((size_t)(am + 1) * (am + 1), 0.0);
+  if ((rc = alloc_level(c, c->d, nullptr))) return fail(rc);
+  if (c->opts.precision == 1) {
+    if ((rc = make_tables(c, c->f.tb))) return fail(rc);
+    if ((rc = alloc_level(c, c->f, c->d.spec))) return fail(rc);
   }
-  c->zinv_blocks = rism::zinv_num_blocks(c->P);
+  if (c->opts.anderson_m > 0)
+    c->D.assign((size_t)(c->opts.anderson_m + 1) * (c->opts.anderson_m + 1), 0.0);
+  c->zinv_blocks = rism::zinv_num_blocks<double>(c->P);
   if ((rc = c->alloc(&c->partial, c->zinv_blocks * kSumWidth))) return fail(rc);
   const int tb = rism::thermo_blocks(c->N);
   if ((rc = c->alloc(&c->tpartial, (size_t)tb * 4 * c->natv))) return fail(rc);
@@ -623,7 +848,7 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   if ((rc = c->alloc(&c->scratch, c->N))) return fail(rc);
   c->hsums.assign(kSumWidth * c->natv, 0.0);
   reset_an2(c);
-  if (hipMemsetAsync(c->T[0], 0, sizeof(double) * nn, c->st) != hipSuccess ||
+  if (hipMemsetAsync(c->d.T[0], 0, sizeof(double) * nn, c->st) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
     return fail(rism::set_error(RISM_ERR_HIP, "initial clear failed"));
   return c;
@@ -634,12 +859,17 @@ void rism_destroy(rism_ctx *ctx) { delete ctx; }
 int rism_setup_potentials(rism_ctx *c) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
   const size_t nn = c->N * c->natv;
+  c->prec = 0;
   reset_an2(c);
   // history slots start at zero (the reference leaves them uninitialised)
   if (c->opts.anderson_m == 0) {
     for (int i = 0; i < 2; ++i) {
-      HIPCHECK(hipMemsetAsync(c->Pslot[i], 0, sizeof(double) * nn, c->st));
-      HIPCHECK(hipMemsetAsync(c->R[c->iR_slot[i]], 0, sizeof(double) * nn, c->st));
+      HIPCHECK(hipMemsetAsync(c->d.Pslot[i], 0, sizeof(double) * nn, c->st));
+      HIPCHECK(hipMemsetAsync(c->d.R[c->iR_slot[i]], 0, sizeof(double) * nn, c->st));
+      if (c->opts.precision == 1) {
+        HIPCHECK(hipMemsetAsync(c->f.Pslot[i], 0, sizeof(float) * nn, c->st));
+        HIPCHECK(hipMemsetAsync(c->f.R[c->iR_slot[i]], 0, sizeof(float) * nn, c->st));
+      }
     }
   }
   if (c->opts.verbose) {
@@ -652,160 +882,92 @@ int rism_setup_potentials(rism_ctx *c) {
   const int do_coul = c->opts.ad_switch != 1;  // cal_Coulomb.cu:23
   {
     KTime kt(c, RISM_K_POT);
-    HIPCHECK(rism::run_potential_r(c->atoms, c->natu, c->P, ikbt, ubeta, do_coul, c->u, c->fr, c->st));
+    HIPCHECK(rism::run_potential_r(c->atoms, c->natu, c->P, ikbt, ubeta, do_coul, c->d.u,
+                                   c->d.fr, c->st));
   }
   if (do_coul && c->natu > 0) {
     KTime kt(c, RISM_K_FK);
-    HIPCHECK(rism::run_fk_half(c->atoms, c->natu, 4 + 2 * c->natv, c->P, ubeta, c->fk, c->st));
+    HIPCHECK(rism::run_fk_half(c->atoms, c->natu, 4 + 2 * c->natv, c->P, ubeta, c->d.fk, c->st));
   } else {
-    HIPCHECK(hipMemsetAsync(c->fk, 0, sizeof(cplx) * (c->N / 2), c->st));
+    HIPCHECK(hipMemsetAsync(c->d.fk, 0, sizeof(cplx) * (c->N / 2), c->st));
   }
   if (c->opts.verbose) std::cout << "synthesizing initial estimate for Tuv ..." << std::endl;
-  HIPCHECK(rism::run_init_t(c->T[c->iT_cur], c->fr, c->P, c->N, c->st));
+  HIPCHECK(rism::run_init_t(c->d.T[c->iT_cur], c->d.fr, c->P, c->N, c->st));
+  if (c->opts.precision == 1) {  // the FP32 level starts from the FP64 set-up
+    HIPCHECK(rism::run_convert(c->d.u, c->f.u, nn, c->st));
+    HIPCHECK(rism::run_convert(c->d.fr, c->f.fr, c->N, c->st));
+    HIPCHECK(rism::run_convert((const double *)c->d.fk, (float *)c->f.fk, c->N, c->st));
+    HIPCHECK(rism::run_convert(c->d.T[c->iT_cur], c->f.T[c->iT_cur], nn, c->st));
+    c->prec = c->last_prec = 1;
+  }
   HIPCHECK(hipStreamSynchronize(c->st));
   return RISM_OK;
 }
 
 int rism_calculate(rism_ctx *c, double *rms) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
-  rism::Params P = c->P;
-  P.mode = 0;
-  double *t = c->T[c->iT_cur];
-  {
-    KTime kt(c, RISM_K_ZFWD);
-    HIPCHECK(rism::run_zfwd(P, t, c->u, c->fr, c->spec, c->tb.twz, c->tb.zpre, c->tb.zpost, c->st));
-  }
-  {
-    KTime kt(c, RISM_K_YFWD);
-    HIPCHECK(rism::run_y(P, c->spec, -1, c->tb.twy, c->tb.prey, c->st));
-  }
-  {
-    KTime kt(c, RISM_K_XOZ);
-    HIPCHECK(rism::run_xoz(P, c->spec, c->fk, c->tb.twx, c->tb.prex, c->knots, c->coef, c->chitab, c->st));
-  }
-  {
-    KTime kt(c, RISM_K_YINV);
-    HIPCHECK(rism::run_y(P, c->spec, +1, c->tb.twy, c->tb.prey, c->st));
-  }
-  const bool andm = c->opts.anderson_m > 0;
-  const int W = andm ? kSumWidth : 6;
-  double *Rout = andm ? c->Rb[c->cur_rb] : c->R[c->iR_cur];
-  {
-    KTime kt(c, RISM_K_ZINV);
-    if (andm) {
-      rism::Hist H{};
-      H.nh = (int)c->hist.size();
-      for (int p = 0; p < H.nh; ++p) H.r[p] = c->Rb[c->hist[p].rb];
-      HIPCHECK(rism::run_zinv_hist(P, c->spec, t, c->u, Rout, H, c->partial, c->tb.twz,
-                                   c->tb.zpre, c->tb.zpost, c->st));
-    } else {
-      const int use_hist = (c->count == 0);
-      HIPCHECK(rism::run_zinv(P, c->spec, t, c->u, Rout, c->R[c->iR_slot[0]],
-                              c->R[c->iR_slot[1]], c->partial, c->tb.twz, c->tb.zpre,
-                              c->tb.zpost, use_hist, c->st));
-    }
-  }
-  const size_t nb = c->zinv_blocks / c->natv;
-  {
-    KTime kt(c, RISM_K_REDUCE);
-    for (int v = 0; v < c->natv; ++v)
-      HIPCHECK(rism::run_reduce_cols(c->partial + (size_t)v * nb * W, nb, W, c->sums + W * v, c->st));
-  }
-  HIPCHECK(hipMemcpyAsync(c->hsums.data(), c->sums, sizeof(double) * W * c->natv,
-                          hipMemcpyDeviceToHost, c->st));
-  HIPCHECK(hipStreamSynchronize(c->st));
-  if (c->ktime) ktime_resolve(c);
-  double ss = 0.0;
-  for (int v = 0; v < c->natv; ++v) ss += c->hsums[W * v];
-  if (andm) {  // weighted dots of the new residual with itself and the history
-    const int m1 = c->opts.anderson_m + 1;
-    auto wdot = [&](int k) {
-      double a = 0.0;
-      for (int v = 0; v < c->natv; ++v) a += c->hsums[W * v + k] / c->rho[v];
-      return a;
-    };
-    c->D[(size_t)c->cur_rb * m1 + c->cur_rb] = wdot(0);
-    for (size_t p = 0; p < c->hist.size(); ++p) {
-      const int b = c->hist[p].rb;
-      c->D[(size_t)c->cur_rb * m1 + b] = c->D[(size_t)b * m1 + c->cur_rb] = wdot(1 + (int)p);
-    }
-  }
-  c->iT_last = c->iT_cur;
-  c->Rlast = Rout;
-  c->fresh = true;
-  c->have_step = true;
-  if (rms) *rms = std::sqrt(ss / ((double)c->N * c->natv));  // cal_rms.cu:24
-  return RISM_OK;
+  return c->prec ? calc_step<float>(c, rms) : calc_step<double>(c, rms);
 }
 
 int rism_anderson(rism_ctx *c) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
-  if (!c->have_step || !c->fresh)
+  if (!c->have_step || !c->fresh || c->last_prec != c->prec)
     return rism::set_error(RISM_ERR_STATE, "rism_anderson needs a preceding rism_calculate");
   c->fresh = false;
-  if (c->opts.anderson_m > 0) return anderson_m_step(c);
-  const size_t nn = c->N * c->natv;
-  const int b = c->binary;
-  const int oldR = c->iR_slot[b];
-  double *t = c->T[c->iT_cur], *R = c->R[c->iR_cur], *tnew = c->T[1 - c->iT_cur];
-  if (c->count > 0) {  // anderson.cu:54-60
-    --c->count;
-    KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_an2_picard(tnew, t, R, c->Pslot[b], c->opts.an2_m, nn, c->st));
-  } else {  // anderson.cu:61-72, cal_theta anderson.cu:81-102
-    double cc[2] = {0, 0}, a[4] = {0, 0, 0, 0};
-    for (int v = 0; v < c->natv; ++v) {
-      const double ir = 1.0 / c->rho[v];
-      const double *s = &c->hsums[6 * v];
-      cc[0] += s[1] * ir;
-      cc[1] += s[2] * ir;
-      a[0] += s[3] * ir;
-      a[1] += s[4] * ir;
-      a[3] += s[5] * ir;
-    }
-    a[2] = a[1];
-    // leg(): Gaussian elimination without pivoting (anderson.cu:105-130)
-    double x[2];
-    {
-      double p = a[0];
-      a[1] /= p;
-      double r = cc[0] /= p;
-      double q = a[2];
-      a[3] -= q * a[1];
-      cc[1] -= q * r;
-      x[1] = cc[1] / a[3];
-      x[0] = cc[0] - a[1] * x[1];
-    }
-    const double s1 = x[0] + c->opts.an2_mp * ((b == 0) ? 0 : 1);
-    const double s2 = x[1] + c->opts.an2_mp * ((b == 0) ? 1 : 0);
-    c->lastx[0] = x[0];
-    c->lastx[1] = x[1];
-    c->lasts[0] = s1;
-    c->lasts[1] = s2;
-    KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_an2_update(tnew, t, R, c->Pslot[0], c->Pslot[1], c->Pslot[b], s1, s2,
-                                  c->opts.an2_m, nn, c->st));
-  }
-  // slot[b] <- (t, R): P_b was rewritten by the kernel, R_b by pointer
-  c->iR_slot[b] = c->iR_cur;
-  c->iR_cur = oldR;
-  c->iT_cur = 1 - c->iT_cur;
-  c->binary = (b == 0) ? 1 : 0;
-  return RISM_OK;
+  if (c->opts.anderson_m > 0)
+    return c->prec ? anderson_m_step<float>(c) : anderson_m_step<double>(c);
+  return c->prec ? an2_step<float>(c) : an2_step<double>(c);
 }
+
+int rism_set_precision(rism_ctx *c, int prec) {
+  if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  if (prec != 0 && prec != 1) return rism::set_error(RISM_ERR_ARG, "precision must be 0 or 1");
+  if (prec == 1 && c->opts.precision != 1)
+    return rism::set_error(RISM_ERR_STATE, "the context was created without the FP32 level");
+  if (prec == c->prec) return RISM_OK;
+  return prec ? convert_state<double, float>(c) : convert_state<float, double>(c);
+}
+
+int rism_get_precision(rism_ctx *c) { return c ? c->prec : -1; }
 
 int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, double *rms_out) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
   if (c->opts.verbose) std::cout << "relaxing 3D UV RISM:" << std::endl;
   bool conver = false, diverge = false;
-  int istep = 0, last = 0;
+  int istep = 0, last = 0, low_steps = 0;
   double rms = 0.0;
+  // mixed precision: the FP32 level iterates until its rms reaches the
+  // hand-over level (or stops improving), then the state moves to FP64 and
+  // the loop goes on there to tol
+  const double hand_rms = std::max(tol, c->opts.mixed_switch);
+  constexpr int kStall = 25;
+  double best = HUGE_VAL;
+  int since_best = 0;
   for (istep = 1; istep <= maxstep; ++istep) {  // iterate.cu:48-72
     int rc = rism_calculate(c, &rms);
     if (rc) return rc;
     last = istep;
     diverge = !std::isfinite(rms);
     if (diverge) break;
+    if (c->prec == 1) {
+      ++low_steps;
+      if (rms < 0.99 * best) {
+        best = rms;
+        since_best = 0;
+      } else {
+        ++since_best;
+      }
+      if (rms <= hand_rms || since_best >= kStall) {
+        if ((rc = rism_set_precision(c, 0))) return rc;
+        if (c->opts.verbose)
+          std::cout << " Step = " << istep << " Reside = " << rms << " (FP32; continuing in FP64)"
+                    << std::endl;
+        continue;  // the next step evaluates the same t in FP64
+      }
+      if ((rc = rism_anderson(c))) return rc;
+      if (c->opts.verbose) std::cout << " Step = " << istep << " Reside = " << rms << std::endl;
+      continue;
+    }
     if (rms <= tol) {
       conver = true;
     } else {
@@ -828,6 +990,7 @@ int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, do
   c->timing.steps = last;
   c->timing.status = st;
   c->timing.rms = rms;
+  c->timing.steps_fp32 = low_steps;
   return RISM_OK;
 }
 
@@ -837,7 +1000,12 @@ int rism_thermo(rism_ctx *c, rism_thermo_out *out) {
   const int tb = rism::thermo_blocks(c->N);
   {
     KTime kt(c, RISM_K_THERMO);
-    HIPCHECK(rism::run_thermo_partial(c->T[c->iT_cur], c->T[c->iT_last], c->u, c->Rlast,
+    if (c->last_prec == 1)
+      HIPCHECK(rism::run_thermo_partial(c->f.T[c->iT_cur], c->f.T[c->iT_last], c->f.u,
+                                        (const float *)c->f.Rlast, c->P, c->N, c->tpartial,
+                                        c->st));
+    else
+      HIPCHECK(rism::run_thermo_partial(c->d.T[c->iT_cur], c->d.T[c->iT_last], c->d.u, c->d.Rlast,
                                       c->P, c->N, c->tpartial, c->st));
   }
   for (int v = 0; v < c->natv; ++v)
@@ -883,28 +1051,49 @@ int rism_get_field(rism_ctx *c, int which, int site, double *host) {
   if (which != RISM_FIELD_FR && (site < 0 || site >= c->natv))
     return rism::set_error(RISM_ERR_ARG, "site out of range");
   const size_t off = (size_t)site * c->N;
+  const bool low = (which == RISM_FIELD_T) ? c->prec == 1 : c->last_prec == 1;
   const double *src = nullptr;
   switch (which) {
     case RISM_FIELD_T:
-      src = c->T[c->iT_cur] + off;
+      if (low) {
+        HIPCHECK(rism::run_convert(c->f.T[c->iT_cur] + off, c->scratch, c->N, c->st));
+        src = c->scratch;
+      } else {
+        src = c->d.T[c->iT_cur] + off;
+      }
       break;
     case RISM_FIELD_U:
-      src = c->u + off;
+      src = c->d.u + off;
       break;
     case RISM_FIELD_FR:
-      src = c->fr;
+      src = c->d.fr;
       break;
     case RISM_FIELD_R:
       if (!c->have_step) return rism::set_error(RISM_ERR_STATE, "no step taken yet");
-      src = c->Rlast + off;
+      if (low) {
+        HIPCHECK(rism::run_convert(c->f.Rlast + off, c->scratch, c->N, c->st));
+        src = c->scratch;
+      } else {
+        src = c->d.Rlast + off;
+      }
       break;
     case RISM_FIELD_G:
     case RISM_FIELD_H:
     case RISM_FIELD_C: {
       const int w = which == RISM_FIELD_G ? 1 : (which == RISM_FIELD_H ? 2 : 3);
       if (w != 1 && !c->have_step) return rism::set_error(RISM_ERR_STATE, "no step taken yet");
-      const double *t = c->T[c->have_step ? c->iT_last : c->iT_cur] + off;
-      HIPCHECK(rism::run_fields(t, c->u + off, c->Rlast + off, c->P, c->N, w, c->scratch, c->st));
+      const int it = c->have_step ? c->iT_last : c->iT_cur;
+      const bool lw = c->have_step ? c->last_prec == 1 : c->prec == 1;
+      // g alone before any step: R is not read
+      if (lw) {
+        const float *R = c->f.Rlast ? c->f.Rlast + off : c->f.T[it] + off;
+        HIPCHECK(rism::run_fields(c->f.T[it] + off, c->f.u + off, R, c->P, c->N, w, c->scratch,
+                                  c->st));
+      } else {
+        const double *R = c->d.Rlast ? c->d.Rlast + off : c->d.T[it] + off;
+        HIPCHECK(rism::run_fields(c->d.T[it] + off, c->d.u + off, R, c->P, c->N, w, c->scratch,
+                                  c->st));
+      }
       src = c->scratch;
       break;
     }
@@ -919,14 +1108,21 @@ int rism_get_field(rism_ctx *c, int which, int site, double *host) {
 int rism_set_field(rism_ctx *c, int which, int site, const double *host) {
   if (!c || !host) return rism::set_error(RISM_ERR_ARG, "null argument");
   if (site < 0 || site >= c->natv) return rism::set_error(RISM_ERR_ARG, "site out of range");
-  double *dst = nullptr;
-  if (which == RISM_FIELD_T)
-    dst = c->T[c->iT_cur] + (size_t)site * c->N;
-  else if (which == RISM_FIELD_U)
-    dst = c->u + (size_t)site * c->N;
-  else
+  const size_t off = (size_t)site * c->N;
+  if (which == RISM_FIELD_T) {
+    if (c->prec == 1) {
+      HIPCHECK(hipMemcpyAsync(c->scratch, host, sizeof(double) * c->N, hipMemcpyHostToDevice, c->st));
+      HIPCHECK(rism::run_convert((const double *)c->scratch, c->f.T[c->iT_cur] + off, c->N, c->st));
+    } else {
+      HIPCHECK(hipMemcpyAsync(c->d.T[c->iT_cur] + off, host, sizeof(double) * c->N,
+                              hipMemcpyHostToDevice, c->st));
+    }
+  } else if (which == RISM_FIELD_U) {
+    HIPCHECK(hipMemcpyAsync(c->d.u + off, host, sizeof(double) * c->N, hipMemcpyHostToDevice, c->st));
+    if (c->f.u) HIPCHECK(rism::run_convert((const double *)c->d.u + off, c->f.u + off, c->N, c->st));
+  } else {
     return rism::set_error(RISM_ERR_ARG, "only T and U can be set");
-  HIPCHECK(hipMemcpyAsync(dst, host, sizeof(double) * c->N, hipMemcpyHostToDevice, c->st));
+  }
   HIPCHECK(hipStreamSynchronize(c->st));
   return RISM_OK;
 }
@@ -1050,7 +1246,7 @@ int transform_ctx(int nx, int ny, int nz, const double box[3], rism_ctx **out) {
     delete c;
     return rism::set_error(RISM_ERR_HIP, "hipStreamCreate failed");
   }
-  if ((rc = make_tables(c))) {
+  if ((rc = make_tables(c, c->d.tb))) {
     delete c;
     return rc;
   }
@@ -1076,9 +1272,9 @@ int rism_k_fft_r2c(int nx, int ny, int nz, const double box[3], const double *in
   HIPCHECK(hipMemcpy(t, in, sizeof(double) * c->N, hipMemcpyHostToDevice));
   rism::Params P = c->P;
   P.mode = 1;
-  HIPCHECK(rism::run_zfwd(P, t, t, t, spec, c->tb.twz, c->tb.zpre, c->tb.zpost, c->st));
-  HIPCHECK(rism::run_y(P, spec, -1, c->tb.twy, c->tb.prey, c->st));
-  HIPCHECK(rism::run_xoz(P, spec, spec, c->tb.twx, c->tb.prex, nullptr, nullptr, nullptr, c->st));
+  HIPCHECK(rism::run_zfwd(P, t, t, t, spec, c->d.tb.twz, c->d.tb.zpre, c->d.tb.zpost, c->st));
+  HIPCHECK(rism::run_y(P, spec, -1, c->d.tb.twy, c->d.tb.prey, c->st));
+  HIPCHECK(rism::run_xoz(P, spec, spec, c->d.tb.twx, c->d.tb.prex, nullptr, nullptr, nullptr, c->st));
   HIPCHECK(rism::run_apply_phase(spec, P, 0, c->st));
   HIPCHECK(hipMemcpyAsync(out, spec, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToHost, c->st));
   HIPCHECK(hipStreamSynchronize(c->st));
@@ -1098,15 +1294,15 @@ int rism_k_fft_c2r(int nx, int ny, int nz, const double box[3], const double *in
   cplx *spec;
   double *partial;
   if ((rc = c->alloc(&h, c->N)) || (rc = c->alloc(&spec, c->N / 2)) ||
-      (rc = c->alloc(&partial, rism::zinv_num_blocks(c->P) * 6)))
+      (rc = c->alloc(&partial, rism::zinv_num_blocks<double>(c->P) * 6)))
     return rc;
   HIPCHECK(hipMemcpy(spec, in, sizeof(cplx) * (c->N / 2), hipMemcpyHostToDevice));
   rism::Params P = c->P;
   P.mode = 2;
   HIPCHECK(rism::run_apply_phase(spec, P, 1, c->st));
-  HIPCHECK(rism::run_xoz(P, spec, spec, c->tb.twx, c->tb.prex, nullptr, nullptr, nullptr, c->st));
-  HIPCHECK(rism::run_y(P, spec, +1, c->tb.twy, c->tb.prey, c->st));
-  HIPCHECK(rism::run_zinv(P, spec, h, h, h, h, h, partial, c->tb.twz, c->tb.zpre, c->tb.zpost, 0, c->st));
+  HIPCHECK(rism::run_xoz(P, spec, spec, c->d.tb.twx, c->d.tb.prex, nullptr, nullptr, nullptr, c->st));
+  HIPCHECK(rism::run_y(P, spec, +1, c->d.tb.twy, c->d.tb.prey, c->st));
+  HIPCHECK(rism::run_zinv(P, spec, h, h, h, h, h, partial, c->d.tb.twz, c->d.tb.zpre, c->d.tb.zpost, 0, c->st));
   HIPCHECK(hipMemcpyAsync(out, h, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->st));
   HIPCHECK(hipStreamSynchronize(c->st));
   return RISM_OK;
@@ -1114,8 +1310,8 @@ int rism_k_fft_c2r(int nx, int ny, int nz, const double box[3], const double *in
 
 int rism_k_get_fk(rism_ctx *c, double *out) {
   if (!c || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
-  cplx *tmp = reinterpret_cast<cplx *>(c->spec);  // borrow the spectrum buffer
-  HIPCHECK(hipMemcpyAsync(tmp, c->fk, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToDevice, c->st));
+  cplx *tmp = c->d.spec;  // borrow the spectrum buffer
+  HIPCHECK(hipMemcpyAsync(tmp, c->d.fk, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToDevice, c->st));
   HIPCHECK(rism::run_apply_phase(tmp, c->P, 0, c->st));
   HIPCHECK(hipMemcpyAsync(out, tmp, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToHost, c->st));
   HIPCHECK(hipStreamSynchronize(c->st));

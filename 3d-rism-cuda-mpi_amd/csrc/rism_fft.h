@@ -17,26 +17,54 @@
 
 namespace rism {
 
-struct cplx {
-  double x, y;
+// Complex value of scalar type Re (double: the FP64 path; float: the FP32
+// bulk of the mixed-precision path).
+template <class Re>
+struct cx {
+  Re x, y;
+};
+using cplx = cx<double>;
+using cplxf = cx<float>;
+
+template <class Re>
+struct ident {
+  using type = Re;
 };
 
+template <class Re>
+__device__ __forceinline__ cx<Re> mkc(Re a, Re b) {
+  return cx<Re>{a, b};
+}
 __device__ __forceinline__ cplx mk(double a, double b) { return cplx{a, b}; }
-__device__ __forceinline__ cplx operator+(cplx a, cplx b) { return cplx{a.x + b.x, a.y + b.y}; }
-__device__ __forceinline__ cplx operator-(cplx a, cplx b) { return cplx{a.x - b.x, a.y - b.y}; }
-__device__ __forceinline__ cplx operator*(double s, cplx a) { return cplx{s * a.x, s * a.y}; }
-__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
-  return cplx{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+template <class Re>
+__device__ __forceinline__ cx<Re> operator+(cx<Re> a, cx<Re> b) {
+  return cx<Re>{a.x + b.x, a.y + b.y};
+}
+template <class Re>
+__device__ __forceinline__ cx<Re> operator-(cx<Re> a, cx<Re> b) {
+  return cx<Re>{a.x - b.x, a.y - b.y};
+}
+template <class Re>
+__device__ __forceinline__ cx<Re> operator*(typename ident<Re>::type s, cx<Re> a) {
+  return cx<Re>{s * a.x, s * a.y};
+}
+template <class Re>
+__device__ __forceinline__ cx<Re> cmul(cx<Re> a, cx<Re> b) {
+  return cx<Re>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 // a * conj(b)
-__device__ __forceinline__ cplx cmulc(cplx a, cplx b) {
-  return cplx{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+template <class Re>
+__device__ __forceinline__ cx<Re> cmulc(cx<Re> a, cx<Re> b) {
+  return cx<Re>{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
 }
-__device__ __forceinline__ cplx conj(cplx a) { return cplx{a.x, -a.y}; }
+template <class Re>
+__device__ __forceinline__ cx<Re> conj(cx<Re> a) {
+  return cx<Re>{a.x, -a.y};
+}
 // multiply by S*i, S = +1 or -1
-template <int S>
-__device__ __forceinline__ cplx mul_si(cplx a) {
-  return S > 0 ? cplx{-a.y, a.x} : cplx{a.y, -a.x};
+template <int S, class Re>
+__device__ __forceinline__ cx<Re> mul_si(cx<Re> a) {
+  return S > 0 ? cx<Re>{-a.y, a.x} : cx<Re>{a.y, -a.x};
 }
 
 // ---- radix-R DFT in registers; S = -1 forward, +1 inverse ----------------
@@ -45,8 +73,9 @@ struct Dft;
 
 template <int S>
 struct Dft<2, S> {
-  __device__ __forceinline__ static void run(cplx *v) {
-    cplx a = v[0], b = v[1];
+  template <class Re>
+  __device__ __forceinline__ static void run(cx<Re> *v) {
+    cx<Re> a = v[0], b = v[1];
     v[0] = a + b;
     v[1] = a - b;
   }
@@ -54,9 +83,10 @@ struct Dft<2, S> {
 
 template <int S>
 struct Dft<4, S> {
-  __device__ __forceinline__ static void run(cplx *v) {
-    cplx s02 = v[0] + v[2], d02 = v[0] - v[2];
-    cplx s13 = v[1] + v[3], d13 = mul_si<S>(v[1] - v[3]);
+  template <class Re>
+  __device__ __forceinline__ static void run(cx<Re> *v) {
+    cx<Re> s02 = v[0] + v[2], d02 = v[0] - v[2];
+    cx<Re> s13 = v[1] + v[3], d13 = mul_si<S>(v[1] - v[3]);
     v[0] = s02 + s13;
     v[2] = s02 - s13;
     v[1] = d02 + d13;
@@ -66,16 +96,17 @@ struct Dft<4, S> {
 
 template <int S>
 struct Dft<8, S> {
-  __device__ __forceinline__ static void run(cplx *v) {
-    constexpr double h = 0.70710678118654752440;
-    cplx e[4] = {v[0], v[2], v[4], v[6]};
-    cplx o[4] = {v[1], v[3], v[5], v[7]};
+  template <class Re>
+  __device__ __forceinline__ static void run(cx<Re> *v) {
+    constexpr Re h = (Re)0.70710678118654752440;
+    cx<Re> e[4] = {v[0], v[2], v[4], v[6]};
+    cx<Re> o[4] = {v[1], v[3], v[5], v[7]};
     Dft<4, S>::run(e);
     Dft<4, S>::run(o);
     // w8^k = exp(S 2 pi i k / 8)
-    cplx o1 = cmul(o[1], cplx{h, S * h});
-    cplx o2 = mul_si<S>(o[2]);
-    cplx o3 = cmul(o[3], cplx{-h, S * h});
+    cx<Re> o1 = cmul(o[1], cx<Re>{h, S * h});
+    cx<Re> o2 = mul_si<S>(o[2]);
+    cx<Re> o3 = cmul(o[3], cx<Re>{-h, S * h});
     v[0] = e[0] + o[0];
     v[4] = e[0] - o[0];
     v[1] = e[1] + o1;
@@ -89,11 +120,12 @@ struct Dft<8, S> {
 
 template <int S>
 struct Dft<3, S> {
-  __device__ __forceinline__ static void run(cplx *v) {
-    constexpr double s3 = 0.86602540378443864676;  // sqrt(3)/2
-    cplx t1 = v[1] + v[2];
-    cplx t2 = v[0] - 0.5 * t1;
-    cplx t3 = mul_si<S>(s3 * (v[1] - v[2]));
+  template <class Re>
+  __device__ __forceinline__ static void run(cx<Re> *v) {
+    constexpr Re s3 = (Re)0.86602540378443864676;  // sqrt(3)/2
+    cx<Re> t1 = v[1] + v[2];
+    cx<Re> t2 = v[0] - (Re)0.5 * t1;
+    cx<Re> t3 = mul_si<S>(s3 * (v[1] - v[2]));
     v[0] = v[0] + t1;
     v[1] = t2 + t3;
     v[2] = t2 - t3;
@@ -102,17 +134,18 @@ struct Dft<3, S> {
 
 template <int S>
 struct Dft<5, S> {
-  __device__ __forceinline__ static void run(cplx *v) {
-    constexpr double c1 = 0.30901699437494742410;   // cos(2pi/5)
-    constexpr double c2 = -0.80901699437494742410;  // cos(4pi/5)
-    constexpr double s1 = 0.95105651629515357212;   // sin(2pi/5)
-    constexpr double s2 = 0.58778525229247312917;   // sin(4pi/5)
-    cplx t1 = v[1] + v[4], t2 = v[2] + v[3];
-    cplx t3 = v[1] - v[4], t4 = v[2] - v[3];
-    cplx b1 = v[0] + c1 * t1 + c2 * t2;
-    cplx b2 = v[0] + c2 * t1 + c1 * t2;
-    cplx d1 = mul_si<S>(s1 * t3 + s2 * t4);
-    cplx d2 = mul_si<S>(s2 * t3 - s1 * t4);
+  template <class Re>
+  __device__ __forceinline__ static void run(cx<Re> *v) {
+    constexpr Re c1 = (Re)0.30901699437494742410;   // cos(2pi/5)
+    constexpr Re c2 = (Re)-0.80901699437494742410;  // cos(4pi/5)
+    constexpr Re s1 = (Re)0.95105651629515357212;   // sin(2pi/5)
+    constexpr Re s2 = (Re)0.58778525229247312917;   // sin(4pi/5)
+    cx<Re> t1 = v[1] + v[4], t2 = v[2] + v[3];
+    cx<Re> t3 = v[1] - v[4], t4 = v[2] - v[3];
+    cx<Re> b1 = v[0] + c1 * t1 + c2 * t2;
+    cx<Re> b2 = v[0] + c2 * t1 + c1 * t2;
+    cx<Re> d1 = mul_si<S>(s1 * t3 + s2 * t4);
+    cx<Re> d2 = mul_si<S>(s2 * t3 - s1 * t4);
     v[0] = v[0] + t1 + t2;
     v[1] = b1 + d1;
     v[4] = b1 - d1;
@@ -170,14 +203,14 @@ struct FftShape {
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
 
 // One Stockham stage.  lds = this column's LDS region; tw = exp(-2 pi i q / L).
-template <int L, int R, int NS, int S, int T, bool FIRST, bool LAST, class Load,
+template <int L, int R, int NS, int S, int T, bool FIRST, bool LAST, class Re, class Load,
           class Store>
-__device__ __forceinline__ void stockham_stage(cplx *lds, int tid,
-                                               const cplx *__restrict__ tw,
+__device__ __forceinline__ void stockham_stage(cx<Re> *lds, int tid,
+                                               const cx<Re> *__restrict__ tw,
                                                Load &load, Store &store) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
-  cplx v[PER][R];
+  cx<Re> v[PER][R];
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
     const int j = tid + b * T;
@@ -194,7 +227,7 @@ __device__ __forceinline__ void stockham_stage(cplx *lds, int tid,
         const int k = j % NS;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const cplx w = tw[r * k * (L / (NS * R))];
+          const cx<Re> w = tw[r * k * (L / (NS * R))];
           v[b][r] = (S < 0) ? cmul(v[b][r], w) : cmulc(v[b][r], w);
         }
       }
@@ -222,8 +255,8 @@ __device__ __forceinline__ void stockham_stage(cplx *lds, int tid,
 }
 
 template <int L, int S, int T, int STAGE, int NS, bool FROM_GLOBAL, bool TO_GLOBAL,
-          class Load, class Store>
-__device__ __forceinline__ void stockham_run(cplx *lds, int tid, const cplx *__restrict__ tw,
+          class Re, class Load, class Store>
+__device__ __forceinline__ void stockham_run(cx<Re> *lds, int tid, const cx<Re> *__restrict__ tw,
                                              Load &load, Store &store) {
   constexpr RadixList rl = make_radices(L);
   if constexpr (STAGE < rl.n) {
@@ -241,8 +274,8 @@ __device__ __forceinline__ void stockham_run(cplx *lds, int tid, const cplx *__r
 // TO_GLOBAL: the last stage calls store(i, v); otherwise the result is left in
 // LDS and the function ends with a barrier.  S = -1 forward, +1 inverse
 // (unnormalised).
-template <int L, int S, bool FROM_GLOBAL, bool TO_GLOBAL, class Load, class Store>
-__device__ __forceinline__ void fft_column(cplx *lds, int tid, const cplx *__restrict__ tw,
+template <int L, int S, bool FROM_GLOBAL, bool TO_GLOBAL, class Re, class Load, class Store>
+__device__ __forceinline__ void fft_column(cx<Re> *lds, int tid, const cx<Re> *__restrict__ tw,
                                            Load &load, Store &store) {
   constexpr int T = FftShape<L>::T;
   stockham_run<L, S, T, 0, 1, FROM_GLOBAL, TO_GLOBAL>(lds, tid, tw, load, store);

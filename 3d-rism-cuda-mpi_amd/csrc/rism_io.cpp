@@ -169,12 +169,22 @@ int rism_read_input(const char *control, const char *structure, int center, int 
   if (!f) return fail(RISM_ERR_IO, "cannot parse line 1 of the input file");
   if (check != "20220301") return fail(RISM_ERR_VERSION, "Error: This input file is for old version.");
   f >> closure;
-  if (closure == "KH")
+  if (closure == "KH") {
     in->closure = RISM_CLOSURE_KH;
-  else if (closure == "HNC")
+  } else if (closure == "HNC") {
     in->closure = RISM_CLOSURE_HNC;
-  else
-    return fail(RISM_ERR_CLOSURE, "Error: Unexpected closure switch ");
+  } else {
+    // PSEn / PSE-n (not in the reference's reader)
+    int n = 0;
+    if (closure.size() > 3 && closure.compare(0, 3, "PSE") == 0) {
+      const std::string num = closure.substr(closure[3] == '-' ? 4 : 3);
+      if (!num.empty() && num.find_first_not_of("0123456789") == std::string::npos &&
+          num.size() <= 2)
+        n = std::atoi(num.c_str());
+    }
+    if (n < 1 || n > 20) return fail(RISM_ERR_CLOSURE, "Error: Unexpected closure switch ");
+    in->closure = RISM_CLOSURE_PSE(n);
+  }
   f >> fsolvent;
   f >> in->convergence >> in->maxstep;
   f >> in->an2_count >> in->an2_m >> in->an2_mp;

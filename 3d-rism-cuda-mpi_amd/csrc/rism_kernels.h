@@ -7,7 +7,11 @@
 
 namespace rism {
 
-struct cplx;
+template <class Re>
+struct cx;
+using cplx = cx<double>;
+using cplxf = cx<float>;
+
 
 // Everything a kernel needs to know about the grid, the solvent sites and
 // the chi spline, passed by value.
@@ -29,59 +33,74 @@ struct Params {
 
 // History of the Anderson(m) accelerator passed to kernels by value.
 #define RISM_MAX_HIST 16
-struct Hist {
-  const double *r[RISM_MAX_HIST];  // residual history R_p, p < nh
+template <class Re>
+struct HistT {
+  const Re *r[RISM_MAX_HIST];  // residual history R_p, p < nh
   int nh;
 };
-struct HistUpdate {
-  const double *p[RISM_MAX_HIST];  // P_p = T_p + beta R_p, p < nh
-  double x[RISM_MAX_HIST];         // mixing coefficients
+template <class Re>
+struct HistUpdateT {
+  const Re *p[RISM_MAX_HIST];  // P_p = T_p + beta R_p, p < nh
+  double x[RISM_MAX_HIST];     // mixing coefficients
   int nh;
 };
 
 bool fft_length_supported(int L);
+template <class Re>
 size_t zinv_num_blocks(const Params &P);
 int xoz_threads(int L, int natv);
 int thermo_blocks(size_t N);
 
-hipError_t run_zfwd(const Params &P, const double *t, const double *u, const double *fr,
-                    cplx *spec, const cplx *twM, const cplx *zpre, const cplx *zpost,
-                    hipStream_t st);
-hipError_t run_y(const Params &P, cplx *spec, int dir, const cplx *twL, const cplx *pre,
+// The hot-loop launchers are instantiated for the FP64 path (Re = double)
+// and the FP32 bulk of the mixed-precision path (Re = float).  Sums of
+// squares, Gram entries and thermodynamic integrals are FP64 in both.
+template <class Re>
+hipError_t run_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<Re> *spec,
+                    const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost, hipStream_t st);
+template <class Re>
+hipError_t run_y(const Params &P, cx<Re> *spec, int dir, const cx<Re> *twL, const cx<Re> *pre,
                  hipStream_t st);
-hipError_t run_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *twL,
-                   const cplx *prex, const double *knots, const double *coef,
+template <class Re>
+hipError_t run_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const cx<Re> *twL,
+                   const cx<Re> *prex, const double *knots, const double *coef,
                    const double *chitab, hipStream_t st);
 hipError_t run_chi_table(const double *knots, const double *coef, const Params &P,
                          double kscale, size_t nshell, double *out, hipStream_t st);
-hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
-                    double *R, const double *R0, const double *R1, double *partial,
-                    const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,
-                    hipStream_t st);
+// AN2 partial sums (6 per workgroup) of the residual pass
+template <class Re>
+hipError_t run_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
+                    const Re *R0, const Re *R1, double *partial, const cx<Re> *twM,
+                    const cx<Re> *zpre, const cx<Re> *zpost, int use_hist, hipStream_t st);
 // Anderson(m) variant: partial sums R.R and R.R_p for the nh history vectors
 // (width 1 + RISM_MAX_HIST per workgroup).
-hipError_t run_zinv_hist(const Params &P, const cplx *spec, const double *t, const double *u,
-                         double *R, const Hist &H, double *partial, const cplx *twM,
-                         const cplx *zpre, const cplx *zpost, hipStream_t st);
-hipError_t run_andm_update(double *tnew, const double *t, const double *R, const HistUpdate &H,
-                           double *Pb, double beta, size_t n, hipStream_t st);
+template <class Re>
+hipError_t run_zinv_hist(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
+                         const HistT<Re> &H, double *partial, const cx<Re> *twM,
+                         const cx<Re> *zpre, const cx<Re> *zpost, hipStream_t st);
+template <class Re>
+hipError_t run_andm_update(Re *tnew, const Re *t, const Re *R, const HistUpdateT<Re> &H, Re *Pb,
+                           double beta, size_t n, hipStream_t st);
 hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out,
                            hipStream_t st);
-hipError_t run_an2_picard(double *tnew, const double *t, const double *R, double *Pb, double m,
-                          size_t n, hipStream_t st);
-hipError_t run_an2_update(double *tnew, const double *t, const double *R, const double *P0,
-                          const double *P1, double *Pb, double s1, double s2, double m, size_t n,
+template <class Re>
+hipError_t run_an2_picard(Re *tnew, const Re *t, const Re *R, Re *Pb, double m, size_t n,
                           hipStream_t st);
+template <class Re>
+hipError_t run_an2_update(Re *tnew, const Re *t, const Re *R, const Re *P0, const Re *P1, Re *Pb,
+                          double s1, double s2, double m, size_t n, hipStream_t st);
+template <class From, class To>
+hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st);
 hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
                            double ubeta, int do_coul, double *u, double *fr, hipStream_t st);
 hipError_t run_fk_half(const double *atoms, int natu, int astride, const Params &P,
                        double ubeta, cplx *fkraw, hipStream_t st);
 hipError_t run_init_t(double *t, const double *fr, const Params &P, size_t N, hipStream_t st);
-hipError_t run_fields(const double *t, const double *u, const double *R, const Params &P,
-                      size_t n, int which, double *out, hipStream_t st);
-hipError_t run_thermo_partial(const double *t, const double *th, const double *u,
-                              const double *R, const Params &P, size_t N, double *partial,
-                              hipStream_t st);
+template <class Re>
+hipError_t run_fields(const Re *t, const Re *u, const Re *R, const Params &P, size_t n,
+                      int which, double *out, hipStream_t st);
+template <class Re>
+hipError_t run_thermo_partial(const Re *t, const Re *th, const Re *u, const Re *R,
+                              const Params &P, size_t N, double *partial, hipStream_t st);
 hipError_t run_apply_phase(cplx *s, const Params &P, int inverse, hipStream_t st);
 
 }  // namespace rism

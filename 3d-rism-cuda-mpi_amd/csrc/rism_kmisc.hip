@@ -46,28 +46,48 @@ __global__ void __launch_bounds__(256)
 // Picard warm-up (newdt0): t' = t + R, P_b = t + m R.
 // Both kernels move two consecutive elements per thread as 16-byte accesses
 // (n = natv * N is even: every grid size is even).
-__global__ void k_an2_picard(double2 *__restrict__ tnew, const double2 *__restrict__ t,
-                             const double2 *__restrict__ R, double2 *__restrict__ Pb,
-                             double mcoef, size_t n2) {
+// Elementwise kernels move 16 B per lane per field (two FP64 or four FP32
+// values).
+template <class Re>
+struct alignas(16) V16 {
+  static constexpr int K = 16 / sizeof(Re);
+  Re a[K];
+};
+
+template <class Re>
+__global__ void k_an2_picard(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
+                             const V16<Re> *__restrict__ R, V16<Re> *__restrict__ Pb, Re mcoef,
+                             size_t nv) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n2) return;
-  const double2 ti = t[i], ri = R[i];
-  tnew[i] = make_double2(ti.x + ri.x, ti.y + ri.y);
-  Pb[i] = make_double2(ti.x + mcoef * ri.x, ti.y + mcoef * ri.y);
+  if (i >= nv) return;
+  const V16<Re> ti = t[i], ri = R[i];
+  V16<Re> a, b;
+#pragma unroll
+  for (int k = 0; k < V16<Re>::K; ++k) {
+    a.a[k] = ti.a[k] + ri.a[k];
+    b.a[k] = ti.a[k] + mcoef * ri.a[k];
+  }
+  tnew[i] = a;
+  Pb[i] = b;
 }
 
 // Pb aliases P0 or P1 (the slot being retired); each element is read before
 // it is written by the same thread.
-__global__ void k_an2_update(double2 *__restrict__ tnew, const double2 *__restrict__ t,
-                             const double2 *__restrict__ R, const double2 *P0, const double2 *P1,
-                             double2 *Pb, double s1, double s2, double mcoef, size_t n2) {
+template <class Re>
+__global__ void k_an2_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
+                             const V16<Re> *__restrict__ R, const V16<Re> *P0,
+                             const V16<Re> *P1, V16<Re> *Pb, Re s1, Re s2, Re mcoef, size_t nv) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n2) return;
-  const double2 ti = t[i], ri = R[i], p0 = P0[i], p1 = P1[i];
-  const double qx = ti.x + mcoef * ri.x, qy = ti.y + mcoef * ri.y;
-  tnew[i] = make_double2(qx + s1 * (p0.x - qx) + s2 * (p1.x - qx),
-                         qy + s1 * (p0.y - qy) + s2 * (p1.y - qy));
-  Pb[i] = make_double2(qx, qy);
+  if (i >= nv) return;
+  const V16<Re> ti = t[i], ri = R[i], p0 = P0[i], p1 = P1[i];
+  V16<Re> a, q;
+#pragma unroll
+  for (int k = 0; k < V16<Re>::K; ++k) {
+    q.a[k] = ti.a[k] + mcoef * ri.a[k];
+    a.a[k] = q.a[k] + s1 * (p0.a[k] - q.a[k]) + s2 * (p1.a[k] - q.a[k]);
+  }
+  tnew[i] = a;
+  Pb[i] = q;
 }
 
 // ---------------------------------------------------------------------------
@@ -76,24 +96,34 @@ __global__ void k_an2_update(double2 *__restrict__ tnew, const double2 *__restri
 // with the retired slot's P overwritten by Q.  Pb may alias one of the P_p;
 // each element is read before the same thread writes it.
 // ---------------------------------------------------------------------------
-__global__ void k_andm_update(double2 *__restrict__ tnew, const double2 *__restrict__ t,
-                              const double2 *__restrict__ R, HistUpdate H, double2 *Pb,
-                              double beta, size_t n2) {
+template <class Re>
+__global__ void k_andm_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
+                              const V16<Re> *__restrict__ R, HistUpdateT<Re> H, V16<Re> *Pb,
+                              Re beta, size_t nv) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n2) return;
-  const double2 ti = t[i], ri = R[i];
-  const double qx = ti.x + beta * ri.x, qy = ti.y + beta * ri.y;
-  double ax = qx, ay = qy;
+  if (i >= nv) return;
+  const V16<Re> ti = t[i], ri = R[i];
+  V16<Re> q, a;
 #pragma unroll
-  for (int q = 0; q < RISM_MAX_HIST; ++q) {
-    if (q < H.nh) {
-      const double2 p = reinterpret_cast<const double2 *>(H.p[q])[i];
-      ax = fma(H.x[q], p.x - qx, ax);
-      ay = fma(H.x[q], p.y - qy, ay);
+  for (int k = 0; k < V16<Re>::K; ++k) a.a[k] = q.a[k] = ti.a[k] + beta * ri.a[k];
+#pragma unroll
+  for (int h = 0; h < RISM_MAX_HIST; ++h) {
+    if (h < H.nh) {
+      const V16<Re> p = reinterpret_cast<const V16<Re> *>(H.p[h])[i];
+      const Re x = (Re)H.x[h];
+#pragma unroll
+      for (int k = 0; k < V16<Re>::K; ++k) a.a[k] = fma(x, p.a[k] - q.a[k], a.a[k]);
     }
   }
-  tnew[i] = make_double2(ax, ay);
-  Pb[i] = make_double2(qx, qy);
+  tnew[i] = a;
+  Pb[i] = q;
+}
+
+// conversion between the FP64 and FP32 copies of a field
+template <class From, class To>
+__global__ void k_convert(const From *__restrict__ in, To *__restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (To)in[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -287,45 +317,51 @@ __global__ void k_init_t(double *__restrict__ t, const double *__restrict__ fr, 
 }
 
 // g = closure(t, u), h = R + g - 1 (the h of the step that produced R)
-__global__ void k_fields(const double *__restrict__ t, const double *__restrict__ u,
-                         const double *__restrict__ R, Params P, size_t n, int which,
+template <class Re>
+__global__ void k_fields(const Re *__restrict__ t, const Re *__restrict__ u,
+                         const Re *__restrict__ R, Params P, size_t n, int which,
                          double *__restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double g = closure_g(P.clos, t[i], u[i]);
+  const double ti = (double)t[i];
+  const double g = closure_g(P.clos, ti, (double)u[i]);
   if (which == 1)
     out[i] = g;
   else if (which == 2)
-    out[i] = R[i] + g - 1.0;
+    out[i] = (double)R[i] + g - 1.0;
   else
-    out[i] = R[i] + g - 1.0 - t[i];
+    out[i] = (double)R[i] + g - 1.0 - ti;
 }
 
 // ---------------------------------------------------------------------------
 // thermodynamics (cal_exchem.cu:55-155, cal_pmv.cu:27-57): per-block FP64
 // sums of SC (closure-matched), HNC, GF integrands and h - t, per site.
 // ---------------------------------------------------------------------------
+template <class Re>
 __global__ void __launch_bounds__(256)
-    k_thermo_partial(const double *__restrict__ t, const double *__restrict__ th,
-                     const double *__restrict__ u, const double *__restrict__ R, Params P,
+    k_thermo_partial(const Re *__restrict__ t, const Re *__restrict__ th,
+                     const Re *__restrict__ u, const Re *__restrict__ R, Params P,
                      size_t N, double *__restrict__ partial) {
   // t: the current t (the integrand's t); th: the t of the step that produced
   // R, from which h = R + g(th, u) - 1 is rebuilt (equal unless the loop
   // stopped at maxstep, after an AN2 update)
   __shared__ double sh[4][256 / 64];
   const int v = blockIdx.y;
-  const double *tv = t + (size_t)v * N, *uv = u + (size_t)v * N, *Rv = R + (size_t)v * N;
-  const double *thv = th + (size_t)v * N;
+  const Re *tv = t + (size_t)v * N, *uv = u + (size_t)v * N, *Rv = R + (size_t)v * N;
+  const Re *thv = th + (size_t)v * N;
   double s[4] = {0, 0, 0, 0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += (size_t)gridDim.x * blockDim.x) {
-    const double tt = tv[i];
-    const double g = closure_g(P.clos, thv[i], uv[i]);
-    const double h = Rv[i] + g - 1.0;
+    const double tt = (double)tv[i];
+    const double g = closure_g(P.clos, (double)thv[i], (double)uv[i]);
+    const double h = (double)Rv[i] + g - 1.0;
     const double ehnc = h * 0.5 * tt - (h - tt);
     const double egf = -(h * 0.5 + 1.0) * (h - tt);
     const double ekh = (h > 0.0) ? egf : ehnc;
-    s[0] += (P.clos == 0) ? ekh : ehnc;
+    // PSE-n (clos = 10 + n): HNC integrand minus (t*)^(n+1)/(n+1)!, t* = -u + t
+    const double epse =
+        (P.clos >= 10) ? ehnc - pse_excess(P.clos - 10, (double)thv[i] - (double)uv[i]) : ehnc;
+    s[0] += (P.clos == 0) ? ekh : epse;
     s[1] += ehnc;
     s[2] += egf;
     s[3] += h - tt;
@@ -368,34 +404,61 @@ hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out
 }
 
 
-hipError_t run_an2_picard(double *tnew, const double *t, const double *R, double *Pb, double m,
-                          size_t n, hipStream_t st) {
-  if (n & 1) return hipErrorInvalidValue;
-  const size_t n2 = n / 2;
-  hipLaunchKernelGGL(k_an2_picard, dim3(nblk(n2, 256)), dim3(256), 0, st, (double2 *)tnew,
-                     (const double2 *)t, (const double2 *)R, (double2 *)Pb, m, n2);
-  return hipGetLastError();
+template <class Re>
+static size_t nvec(size_t n) {
+  return n / V16<Re>::K;
 }
-
-hipError_t run_an2_update(double *tnew, const double *t, const double *R, const double *P0,
-                          const double *P1, double *Pb, double s1, double s2, double m, size_t n,
+template <class Re>
+hipError_t run_an2_picard(Re *tnew, const Re *t, const Re *R, Re *Pb, double m, size_t n,
                           hipStream_t st) {
-  if (n & 1) return hipErrorInvalidValue;
-  const size_t n2 = n / 2;
-  hipLaunchKernelGGL(k_an2_update, dim3(nblk(n2, 256)), dim3(256), 0, st, (double2 *)tnew,
-                     (const double2 *)t, (const double2 *)R, (const double2 *)P0,
-                     (const double2 *)P1, (double2 *)Pb, s1, s2, m, n2);
+  if (n % V16<Re>::K) return hipErrorInvalidValue;
+  const size_t nv = nvec<Re>(n);
+  hipLaunchKernelGGL(k_an2_picard<Re>, dim3(nblk(nv, 256)), dim3(256), 0, st, (V16<Re> *)tnew,
+                     (const V16<Re> *)t, (const V16<Re> *)R, (V16<Re> *)Pb, (Re)m, nv);
   return hipGetLastError();
 }
 
-hipError_t run_andm_update(double *tnew, const double *t, const double *R, const HistUpdate &H,
-                           double *Pb, double beta, size_t n, hipStream_t st) {
-  if ((n & 1) || H.nh < 0 || H.nh > RISM_MAX_HIST) return hipErrorInvalidValue;
-  const size_t n2 = n / 2;
-  hipLaunchKernelGGL(k_andm_update, dim3(nblk(n2, 256)), dim3(256), 0, st, (double2 *)tnew,
-                     (const double2 *)t, (const double2 *)R, H, (double2 *)Pb, beta, n2);
+template <class Re>
+hipError_t run_an2_update(Re *tnew, const Re *t, const Re *R, const Re *P0, const Re *P1, Re *Pb,
+                          double s1, double s2, double m, size_t n, hipStream_t st) {
+  if (n % V16<Re>::K) return hipErrorInvalidValue;
+  const size_t nv = nvec<Re>(n);
+  hipLaunchKernelGGL(k_an2_update<Re>, dim3(nblk(nv, 256)), dim3(256), 0, st, (V16<Re> *)tnew,
+                     (const V16<Re> *)t, (const V16<Re> *)R, (const V16<Re> *)P0,
+                     (const V16<Re> *)P1, (V16<Re> *)Pb, (Re)s1, (Re)s2, (Re)m, nv);
   return hipGetLastError();
 }
+
+template <class Re>
+hipError_t run_andm_update(Re *tnew, const Re *t, const Re *R, const HistUpdateT<Re> &H, Re *Pb,
+                           double beta, size_t n, hipStream_t st) {
+  if ((n % V16<Re>::K) || H.nh < 0 || H.nh > RISM_MAX_HIST) return hipErrorInvalidValue;
+  const size_t nv = nvec<Re>(n);
+  hipLaunchKernelGGL(k_andm_update<Re>, dim3(nblk(nv, 256)), dim3(256), 0, st, (V16<Re> *)tnew,
+                     (const V16<Re> *)t, (const V16<Re> *)R, H, (V16<Re> *)Pb, (Re)beta, nv);
+  return hipGetLastError();
+}
+
+template <class From, class To>
+hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL((k_convert<From, To>), dim3(nblk(n, 256)), dim3(256), 0, st, in, out, n);
+  return hipGetLastError();
+}
+
+#define INST(Re)                                                                              \
+  template hipError_t run_an2_picard<Re>(Re *, const Re *, const Re *, Re *, double, size_t,  \
+                                         hipStream_t);                                        \
+  template hipError_t run_an2_update<Re>(Re *, const Re *, const Re *, const Re *, const Re *, \
+                                         Re *, double, double, double, size_t, hipStream_t);  \
+  template hipError_t run_andm_update<Re>(Re *, const Re *, const Re *,                       \
+                                          const HistUpdateT<Re> &, Re *, double, size_t,      \
+                                          hipStream_t);                                       \
+  template hipError_t run_fields<Re>(const Re *, const Re *, const Re *, const Params &,      \
+                                     size_t, int, double *, hipStream_t);                     \
+  template hipError_t run_thermo_partial<Re>(const Re *, const Re *, const Re *, const Re *,  \
+                                             const Params &, size_t, double *, hipStream_t);
+template hipError_t run_convert<double, float>(const double *, float *, size_t, hipStream_t);
+template hipError_t run_convert<float, double>(const float *, double *, size_t, hipStream_t);
 
 hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
                            double ubeta, int do_coul, double *u, double *fr, hipStream_t st) {
@@ -433,9 +496,11 @@ hipError_t run_init_t(double *t, const double *fr, const Params &P, size_t N, hi
   return hipGetLastError();
 }
 
-hipError_t run_fields(const double *t, const double *u, const double *R, const Params &P,
+template <class Re>
+hipError_t run_fields(const Re *t, const Re *u, const Re *R, const Params &P,
                       size_t n, int which, double *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_fields, dim3(nblk(n, 256)), dim3(256), 0, st, t, u, R, P, n, which, out);
+  hipLaunchKernelGGL(k_fields<Re>, dim3(nblk(n, 256)), dim3(256), 0, st, t, u, R, P, n, which,
+                     out);
   return hipGetLastError();
 }
 
@@ -444,11 +509,12 @@ int thermo_blocks(size_t N) {
   return (int)(b < 1024 ? b : 1024);
 }
 
-hipError_t run_thermo_partial(const double *t, const double *th, const double *u,
-                              const double *R, const Params &P, size_t N, double *partial,
+template <class Re>
+hipError_t run_thermo_partial(const Re *t, const Re *th, const Re *u,
+                              const Re *R, const Params &P, size_t N, double *partial,
                               hipStream_t st) {
   dim3 grid(thermo_blocks(N), P.natv);
-  hipLaunchKernelGGL(k_thermo_partial, grid, dim3(256), 0, st, t, th, u, R, P, N, partial);
+  hipLaunchKernelGGL(k_thermo_partial<Re>, grid, dim3(256), 0, st, t, th, u, R, P, N, partial);
   return hipGetLastError();
 }
 
@@ -458,5 +524,9 @@ hipError_t run_apply_phase(cplx *s, const Params &P, int inverse, hipStream_t st
   return hipGetLastError();
 }
 
+
+INST(double)
+INST(float)
+#undef INST
 
 }  // namespace rism

@@ -57,29 +57,29 @@ struct ChiSpline {
 // (mqvfk + oz, calculate_cuda.cu:32-52).  off[k] is the LDS offset of point
 // k in site 0's column; site w sits w*CS further.  NVM is a compile-time
 // bound on the site count so the per-site values stay in registers.
-template <int NVM, int NP, class Chi>
-__device__ __forceinline__ void oz_points(cplx *lds, int CS, const int (&off)[NP],
-                                          const cplx (&fk)[NP], const Params &P, int nv,
+template <int NVM, int NP, class Re, class Chi>
+__device__ __forceinline__ void oz_points(cx<Re> *lds, int CS, const int (&off)[NP],
+                                          const cx<Re> (&fk)[NP], const Params &P, int nv,
                                           const Chi &X) {
-  cplx c[NP][NVM];
+  cx<Re> c[NP][NVM];
 #pragma unroll
   for (int w = 0; w < NVM; ++w) {
     if (w < nv) {
 #pragma unroll
-      for (int k = 0; k < NP; ++k) c[k][w] = lds[off[k] + w * CS] - P.qv[w] * fk[k];
+      for (int k = 0; k < NP; ++k) c[k][w] = lds[off[k] + w * CS] - (Re)P.qv[w] * fk[k];
     }
   }
   // one output site at a time keeps the live chi values to one row of NVM
 #pragma unroll 1
   for (int v = 0; v < NVM; ++v) {
     if (v < nv) {
-      cplx a[NP];
+      cx<Re> a[NP];
 #pragma unroll
-      for (int k = 0; k < NP; ++k) a[k] = mk(0.0, 0.0);
+      for (int k = 0; k < NP; ++k) a[k] = cx<Re>{0, 0};
 #pragma unroll
       for (int w = 0; w < NVM; ++w) {
         if (w < nv) {
-          const double x = X(v * nv + w);
+          const Re x = (Re)X(v * nv + w);
 #pragma unroll
           for (int k = 0; k < NP; ++k) a[k] = a[k] + x * c[k][w];
         }
@@ -93,9 +93,9 @@ __device__ __forceinline__ void oz_points(cplx *lds, int CS, const int (&off)[NP
 // NVMAX (4 or 8) bounds the site counts compiled into one kernel: the
 // register allocation follows the largest case, so natv <= 4 gets a kernel
 // without the 5..8-site paths.
-template <int NVMAX, int NP, class Chi>
-__device__ __forceinline__ void oz_dispatch(cplx *lds, int CS, const int (&off)[NP],
-                                            const cplx (&fk)[NP], const Params &P,
+template <int NVMAX, int NP, class Re, class Chi>
+__device__ __forceinline__ void oz_dispatch(cx<Re> *lds, int CS, const int (&off)[NP],
+                                            const cx<Re> (&fk)[NP], const Params &P,
                                             const Chi &X) {
   switch (P.natv) {
     case 1: oz_points<1>(lds, CS, off, fk, P, 1, X); break;
@@ -115,12 +115,12 @@ __device__ __forceinline__ void oz_dispatch(cplx *lds, int CS, const int (&off)[
   }
 }
 
-template <int NVMAX, class Chi>
-__device__ __forceinline__ void oz_row(cplx *lds, int CS, int jx, int L, const cplx *fkr,
+template <int NVMAX, class Re, class Chi>
+__device__ __forceinline__ void oz_row(cx<Re> *lds, int CS, int jx, int L, const cx<Re> *fkr,
                                        const Params &P, const Chi &X) {
   const int jm = L - 1 - jx;
   const int off[2] = {lpad(jx), lpad(jm)};
-  const cplx fk[2] = {fkr[jx], fkr[jm]};
+  const cx<Re> fk[2] = {fkr[jx], fkr[jm]};
   oz_dispatch<NVMAX, 2>(lds, CS, off, fk, P, X);
 }
 
@@ -128,14 +128,18 @@ __device__ __forceinline__ void oz_row(cplx *lds, int CS, int jx, int L, const c
 // natv * T threads.  SHELL selects the chi source at compile time (the
 // cubic-box shell table or the spline at |k|) and NVMAX bounds the site
 // count, so each instantiation carries only the registers it needs.
-template <int L, int NVMAX, bool SHELL>
+// dynamic LDS of the x passes, shared by all instantiations
+extern __shared__ double xsm[];
+
+template <int L, int NVMAX, bool SHELL, class Re>
 __global__ void __launch_bounds__(512)
-    k_xpass_oz(cplx *__restrict__ spec, const cplx *__restrict__ fkraw, Params P,
-               const cplx *__restrict__ twL, const cplx *__restrict__ prex,
+    k_xpass_oz(cx<Re> *__restrict__ spec, const cx<Re> *__restrict__ fkraw, Params P,
+               const cx<Re> *__restrict__ twL, const cx<Re> *__restrict__ prex,
                const double *__restrict__ knots, const double *__restrict__ coef,
                const double *__restrict__ chitab) {
+  using C = cx<Re>;
   constexpr int T = FftShape<L>::T, CS = FftShape<L>::CS;
-  extern __shared__ cplx lds[];
+  C *lds = reinterpret_cast<C *>(xsm);
   const int tid = threadIdx.x;
   const int nv = P.natv;
   const int v = tid / T, j = tid % T;
@@ -143,17 +147,17 @@ __global__ void __launch_bounds__(512)
   const int M = P.nz / 2;
   const size_t rowstride = (size_t)M * P.ny * L;  // site stride in spec
   const size_t rowoff = ((size_t)m * P.ny + y) * L;
-  cplx *row = spec + (size_t)v * rowstride + rowoff;
-  cplx *mylds = lds + v * CS;
-  auto load = [&](int i) -> cplx { return cmul(row[i], prex[i]); };
-  auto store = [&](int i, cplx vv) { row[i] = cmulc(vv, prex[i]); };
-  auto nostore = [&](int, cplx) {};
+  C *row = spec + (size_t)v * rowstride + rowoff;
+  C *mylds = lds + v * CS;
+  auto load = [&](int i) -> C { return cmul(row[i], prex[i]); };
+  auto store = [&](int i, C vv) { row[i] = cmulc(vv, prex[i]); };
+  auto nostore = [&](int, C) {};
   fft_column<L, -1, true, false>(mylds, j, twL, load, nostore);
 
   // OZ at every k of the row, the mirror points (kx, -kx) together
   const int nthr = nv * T;
   const int nv2 = nv * nv;
-  const cplx *fkr = fkraw + rowoff;
+  const C *fkr = fkraw + rowoff;
   for (int jx = tid; jx < L / 2; jx += nthr) {
     if constexpr (SHELL) {
       const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates
@@ -181,7 +185,7 @@ __global__ void __launch_bounds__(512)
     k_xpass_plain(cplx *__restrict__ spec, Params P, const cplx *__restrict__ twL,
                   const cplx *__restrict__ prex) {
   constexpr int T = FftShape<L>::T, CS = FftShape<L>::CS;
-  extern __shared__ cplx lds[];
+  cplx *lds = reinterpret_cast<cplx *>(xsm);
   const int v = threadIdx.x / T, j = threadIdx.x % T;
   const size_t rowoff = ((size_t)blockIdx.y * P.ny + blockIdx.x) * L;
   cplx *row = spec + (size_t)v * (P.nz / 2) * P.ny * L + rowoff;
@@ -204,23 +208,24 @@ __global__ void k_chi_table(const double *__restrict__ knots, const double *__re
   for (int pw = 0; pw < nv2; ++pw) out[i * nv2 + pw] = X(pw);
 }
 
-template <int L>
-static void launch_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *twL,
-                       const cplx *prex, const double *knots, const double *coef,
+template <int L, class Re>
+static void launch_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const cx<Re> *twL,
+                       const cx<Re> *prex, const double *knots, const double *coef,
                        const double *chitab, hipStream_t st) {
-  if (P.mode == 1 || P.mode == 2) {
-    dim3 grid(P.ny, P.nz / 2);
-    const int threads = P.natv * FftShape<L>::T;
-    const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cplx);
-    if (P.mode == 1)
-      hipLaunchKernelGGL((k_xpass_plain<L, -1>), grid, dim3(threads), lds, st, spec, P, twL, prex);
-    else
-      hipLaunchKernelGGL((k_xpass_plain<L, 1>), grid, dim3(threads), lds, st, spec, P, twL, prex);
-    return;
-  }
   dim3 grid(P.ny, P.nz / 2);
   const int threads = P.natv * FftShape<L>::T;
-  const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cplx);
+  const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cx<Re>);
+  if constexpr (sizeof(Re) == sizeof(double)) {
+    if (P.mode == 1 || P.mode == 2) {
+      if (P.mode == 1)
+        hipLaunchKernelGGL((k_xpass_plain<L, -1>), grid, dim3(threads), lds, st, spec, P, twL,
+                           prex);
+      else
+        hipLaunchKernelGGL((k_xpass_plain<L, 1>), grid, dim3(threads), lds, st, spec, P, twL,
+                           prex);
+      return;
+    }
+  }
   auto go = [&](auto kern) {
     if (lds > 65536)
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
@@ -229,9 +234,9 @@ static void launch_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cpl
                        chitab);
   };
   if (P.natv <= 4)
-    P.shell ? go(k_xpass_oz<L, 4, true>) : go(k_xpass_oz<L, 4, false>);
+    P.shell ? go(k_xpass_oz<L, 4, true, Re>) : go(k_xpass_oz<L, 4, false, Re>);
   else
-    P.shell ? go(k_xpass_oz<L, 8, true>) : go(k_xpass_oz<L, 8, false>);
+    P.shell ? go(k_xpass_oz<L, 8, true, Re>) : go(k_xpass_oz<L, 8, false, Re>);
 }
 int xoz_threads(int L, int natv) {
   switch (L) {
@@ -244,8 +249,9 @@ int xoz_threads(int L, int natv) {
       return 0;
   }
 }
-hipError_t run_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *twL,
-                   const cplx *prex, const double *knots, const double *coef,
+template <class Re>
+hipError_t run_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const cx<Re> *twL,
+                   const cx<Re> *prex, const double *knots, const double *coef,
                    const double *chitab, hipStream_t st) {
   switch (P.nx) {
 #define X(n)                                                           \
@@ -259,6 +265,12 @@ hipError_t run_xoz(const Params &P, cplx *spec, const cplx *fkraw, const cplx *t
   }
   return hipGetLastError();
 }
+template hipError_t run_xoz<double>(const Params &, cplx *, const cplx *, const cplx *,
+                                    const cplx *, const double *, const double *, const double *,
+                                    hipStream_t);
+template hipError_t run_xoz<float>(const Params &, cplxf *, const cplxf *, const cplxf *,
+                                   const cplxf *, const double *, const double *,
+                                   const double *, hipStream_t);
 hipError_t run_chi_table(const double *knots, const double *coef, const Params &P,
                           double kscale, size_t nshell, double *out, hipStream_t st) {
   hipLaunchKernelGGL(k_chi_table, dim3((unsigned)((nshell + 255) / 256)), dim3(256), 0, st, knots,

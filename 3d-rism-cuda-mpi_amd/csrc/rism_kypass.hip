@@ -8,37 +8,39 @@ namespace rism {
 // ---------------------------------------------------------------------------
 // Forward: pre-twiddle a_n = (-1)^n e^{-i pi n/ny}, DFT.  Inverse: DFT^-1
 // (unnormalised), post-twiddle conj(a_n).  The output phases are "raw".
-template <int L, int B, int S>
+template <int L, int B, int S, class Re>
 __global__ void __launch_bounds__(B *FftShape<L>::T)
-    k_ypass(cplx *__restrict__ spec, int nx, const cplx *__restrict__ twL,
-            const cplx *__restrict__ pre) {
+    k_ypass(cx<Re> *__restrict__ spec, int nx, const cx<Re> *__restrict__ twL,
+            const cx<Re> *__restrict__ pre) {
+  using C = cx<Re>;
   constexpr int CS = FftShape<L>::CS;
-  __shared__ cplx lds[B * CS];
+  __shared__ C lds[B * CS];
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   const int x = blockIdx.x * B + c;
   const bool ok = x < nx;
-  cplx *col = spec + (size_t)blockIdx.y * L * nx + (ok ? x : 0);
-  auto load = [&](int i) -> cplx {
-    if (!ok) return mk(0.0, 0.0);
-    const cplx vv = col[(size_t)i * nx];
+  C *col = spec + (size_t)blockIdx.y * L * nx + (ok ? x : 0);
+  auto load = [&](int i) -> C {
+    if (!ok) return C{0, 0};
+    const C vv = col[(size_t)i * nx];
     return (S < 0) ? cmul(vv, pre[i]) : vv;
   };
-  auto store = [&](int i, cplx vv) {
+  auto store = [&](int i, C vv) {
     if (ok) col[(size_t)i * nx] = (S < 0) ? vv : cmulc(vv, pre[i]);
   };
   fft_column<L, S, true, true>(lds + c * CS, j, twL, load, store);
 }
 
-template <int L, int S>
-static void launch_y(const Params &P, cplx *spec, int nplanes, const cplx *twL,
-                     const cplx *pre, hipStream_t st) {
-  constexpr int B = tile_cols<L>();
+template <int L, int S, class Re>
+static void launch_y(const Params &P, cx<Re> *spec, int nplanes, const cx<Re> *twL,
+                     const cx<Re> *pre, hipStream_t st) {
+  constexpr int B = tile_cols<L, Re>();
   dim3 grid((P.nx + B - 1) / B, nplanes);
-  hipLaunchKernelGGL((k_ypass<L, B, S>), grid, dim3(B * FftShape<L>::T), 0, st, spec, P.nx,
+  hipLaunchKernelGGL((k_ypass<L, B, S, Re>), grid, dim3(B * FftShape<L>::T), 0, st, spec, P.nx,
                      twL, pre);
 }
-hipError_t run_y(const Params &P, cplx *spec, int dir, const cplx *twL, const cplx *pre,
+template <class Re>
+hipError_t run_y(const Params &P, cx<Re> *spec, int dir, const cx<Re> *twL, const cx<Re> *pre,
                  hipStream_t st) {
   const int nplanes = P.natv * (P.nz / 2);
   switch (P.ny) {
@@ -56,5 +58,9 @@ hipError_t run_y(const Params &P, cplx *spec, int dir, const cplx *twL, const cp
   }
   return hipGetLastError();
 }
+template hipError_t run_y<double>(const Params &, cplx *, int, const cplx *, const cplx *,
+                                  hipStream_t);
+template hipError_t run_y<float>(const Params &, cplxf *, int, const cplxf *, const cplxf *,
+                                 hipStream_t);
 
 }  // namespace rism

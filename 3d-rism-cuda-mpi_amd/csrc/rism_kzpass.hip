@@ -14,14 +14,15 @@ namespace rism {
 // M-point DFT, G(m) = E(m) + w_m O(m) with E, O from Z(m) and Z(M-1-m) and
 // w_m = e^{-2 pi i (m+1/2)/nz}: the z-DFT at the positive half-shifted
 // frequencies.
-template <int M, int B>
+template <int M, int B, class Re>
 __global__ void __launch_bounds__(B *FftShape<M>::T)
-    k_zfwd_closure(const double *__restrict__ t, const double *__restrict__ u,
-                   const double *__restrict__ fr, Params P, cplx *__restrict__ spec,
-                   const cplx *__restrict__ twM, const cplx *__restrict__ zpre,
-                   const cplx *__restrict__ zpost) {
+    k_zfwd_closure(const Re *__restrict__ t, const Re *__restrict__ u,
+                   const Re *__restrict__ fr, Params P, cx<Re> *__restrict__ spec,
+                   const cx<Re> *__restrict__ twM, const cx<Re> *__restrict__ zpre,
+                   const cx<Re> *__restrict__ zpost) {
+  using C = cx<Re>;
   constexpr int T = FftShape<M>::T, CS = FftShape<M>::CS;
-  __shared__ cplx lds[B * CS];
+  __shared__ C lds[B * CS];
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   // logical workgroup order: site fastest, then x tile, then y (sites of
@@ -35,35 +36,35 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   const bool ok = x < P.nx;
   const size_t plane = (size_t)P.nx * P.ny;
   const size_t N = plane * (size_t)P.nz;
-  const double *tv = t + (size_t)v * N;
-  const double *uv = u + (size_t)v * N;
-  const double q = P.qv[v];
+  const Re *tv = t + (size_t)v * N;
+  const Re *uv = u + (size_t)v * N;
+  const Re q = (Re)P.qv[v];
   const size_t base = (size_t)y * P.nx + (ok ? x : 0);
   const int clos = P.clos;
-  const double dv = P.dv;
-  cplx *col = lds + c * CS;
+  const Re dv = (Re)P.dv;
+  C *col = lds + c * CS;
 
-  auto cval = [&](int z) -> double {
+  auto cval = [&](int z) -> Re {
     const size_t i = base + (size_t)z * plane;
-    const double tt = tv[i];
+    const Re tt = tv[i];
     if (P.mode != 0) return tt * dv;  // plain transform of t
-    const double g = closure_g(clos, tt, uv[i]);
-    return (g - 1.0 - tt + q * fr[i]) * dv;  // trm1mt, times the forward dv
+    const Re g = closure_g(clos, tt, uv[i]);
+    return (g - (Re)1 - tt + q * fr[i]) * dv;  // trm1mt, times the forward dv
   };
-  auto load = [&](int p) -> cplx {
-    if (!ok) return mk(0.0, 0.0);
-    return cmul(mk(cval(2 * p), cval(2 * p + 1)), zpre[p]);
+  auto load = [&](int p) -> C {
+    if (!ok) return C{0, 0};
+    return cmul(C{cval(2 * p), cval(2 * p + 1)}, zpre[p]);
   };
-  auto store = [&](int, cplx) {};
+  auto store = [&](int, C) {};
   fft_column<M, -1, true, false>(col, j, twM, load, store);
 
-  cplx *out = spec + ((size_t)v * M * P.ny + y) * P.nx + x;
+  C *out = spec + ((size_t)v * M * P.ny + y) * P.nx + x;
   const size_t mstride = (size_t)P.ny * P.nx;
   for (int m = j; m < M; m += T) {
-    const cplx a = col[lpad(m)];
-    const cplx b = conj(col[lpad(M - 1 - m)]);
-    const cplx E = 0.5 * (a + b);
-    const cplx O = mul_si<-1>(0.5 * (a - b));
+    const C a = col[lpad(m)];
+    const C b = conj(col[lpad(M - 1 - m)]);
+    const C E = (Re)0.5 * (a + b);
+    const C O = mul_si<-1>((Re)0.5 * (a - b));
     if (ok) out[(size_t)m * mstride] = E + cmul(zpost[m], O);
   }
 }
@@ -80,20 +81,23 @@ struct ZinvSums {
   static constexpr int W = HM == 0 ? 6 : 1 + RISM_MAX_HIST;
 };
 
-template <int M, int B, int HM>
+template <int M, int B, int HM, class Re>
 __global__ void __launch_bounds__(B *FftShape<M>::T)
-    k_zinv_resid(const cplx *__restrict__ spec, const double *__restrict__ t,
-                 const double *__restrict__ u, double *__restrict__ R,
-                 const double *__restrict__ R0, const double *__restrict__ R1, Hist H,
-                 Params P, double *__restrict__ partial, const cplx *__restrict__ twM,
-                 const cplx *__restrict__ zpre, const cplx *__restrict__ zpost, int use_hist) {
+    k_zinv_resid(const cx<Re> *__restrict__ spec, const Re *__restrict__ t,
+                 const Re *__restrict__ u, Re *__restrict__ R, const Re *__restrict__ R0,
+                 const Re *__restrict__ R1, HistT<Re> H, Params P, double *__restrict__ partial,
+                 const cx<Re> *__restrict__ twM, const cx<Re> *__restrict__ zpre,
+                 const cx<Re> *__restrict__ zpost, int use_hist) {
+  using C = cx<Re>;
   constexpr int T = FftShape<M>::T, CS = FftShape<M>::CS;
   constexpr int NT = B * T;
   constexpr int W = ZinvSums<HM>::W;
-  // the LDS tile doubles as the reduction buffer (NT * W doubles when the
-  // workgroup is not made of whole waves)
-  constexpr int LDSN = (B * CS > (NT * W + 1) / 2) ? B * CS : (NT * W + 1) / 2;
-  __shared__ cplx lds[LDSN];
+  // the LDS tile doubles as the reduction buffer: W doubles per wave, or per
+  // thread when the workgroup is not made of whole waves
+  constexpr int NRED = (NT % 64 == 0) ? NT / 64 : NT;
+  constexpr int NC = (int)((NRED * W * sizeof(double) + sizeof(C) - 1) / sizeof(C));
+  constexpr int LDSN = (B * CS > NC) ? B * CS : NC;
+  __shared__ C lds[LDSN];
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   const int ntx = (P.nx + B - 1) / B;
@@ -105,49 +109,51 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   const bool ok = x < P.nx;
   const size_t plane = (size_t)P.nx * P.ny;
   const size_t N = plane * (size_t)P.nz;
-  cplx *col = lds + c * CS;
-  const cplx *in = spec + ((size_t)v * M * P.ny + y) * P.nx + (ok ? x : 0);
-  for (int m = j; m < M; m += T) col[lpad(m)] = ok ? in[(size_t)m * plane] : mk(0.0, 0.0);
+  C *col = lds + c * CS;
+  const C *in = spec + ((size_t)v * M * P.ny + y) * P.nx + (ok ? x : 0);
+  for (int m = j; m < M; m += T) col[lpad(m)] = ok ? in[(size_t)m * plane] : C{0, 0};
   __syncthreads();
   // G(m) -> Z(m) = E(m) + i O(m), pairs (m, M-1-m) handled by one thread
   for (int m = j; 2 * m <= M - 1; m += T) {
     const int mm = M - 1 - m;
-    const cplx a = col[lpad(m)], b = col[lpad(mm)];
-    const cplx Em = 0.5 * (a + conj(b));
-    const cplx Om = cmulc(0.5 * (a - conj(b)), zpost[m]);
-    const cplx Emm = 0.5 * (b + conj(a));
-    const cplx Omm = cmulc(0.5 * (b - conj(a)), zpost[mm]);
+    const C a = col[lpad(m)], b = col[lpad(mm)];
+    const C Em = (Re)0.5 * (a + conj(b));
+    const C Om = cmulc((Re)0.5 * (a - conj(b)), zpost[m]);
+    const C Emm = (Re)0.5 * (b + conj(a));
+    const C Omm = cmulc((Re)0.5 * (b - conj(a)), zpost[mm]);
     col[lpad(m)] = Em + mul_si<1>(Om);
     if (mm != m) col[lpad(mm)] = Emm + mul_si<1>(Omm);
   }
   __syncthreads();
 
-  const double *tv = t + (size_t)v * N;
-  const double *uv = u + (size_t)v * N;
-  double *Rv = R + (size_t)v * N;
-  const double *R0v = R0 + (size_t)v * N;
-  const double *R1v = R1 + (size_t)v * N;
+  const Re *tv = t + (size_t)v * N;
+  const Re *uv = u + (size_t)v * N;
+  Re *Rv = R + (size_t)v * N;
+  const Re *R0v = R0 + (size_t)v * N;
+  const Re *R1v = R1 + (size_t)v * N;
   const size_t base = (size_t)y * P.nx + (ok ? x : 0);
   const int clos = P.clos;
-  const double scale = P.inv_scale;
+  const Re scale = (Re)P.inv_scale;
   double s[W];
 #pragma unroll
   for (int k = 0; k < W; ++k) s[k] = 0.0;
   const int plain = P.mode != 0;
   const size_t voff = (size_t)v * N;
-  auto resid = [&](int z, double h) {
+  auto resid = [&](int z, Re h) {
     const size_t i = base + (size_t)z * plane;
     if (plain) {  // plain inverse transform: write h itself
       Rv[i] = h;
       return;
     }
-    const double g = closure_g(clos, tv[i], uv[i]);
-    const double r = h + 1.0 - g;  // tr (calculate_cuda.cu:58)
-    Rv[i] = r;
+    const Re g = closure_g(clos, tv[i], uv[i]);
+    const Re rr = h + (Re)1 - g;  // tr (calculate_cuda.cu:58)
+    Rv[i] = rr;
+    // the sums are FP64 whatever the bulk type
+    const double r = (double)rr;
     s[0] += r * r;
     if constexpr (HM == 0) {
       if (use_hist) {
-        const double d0 = r - R0v[i], d1 = r - R1v[i];
+        const double d0 = r - (double)R0v[i], d1 = r - (double)R1v[i];
         s[1] += r * d0;
         s[2] += r * d1;
         s[3] += d0 * d0;
@@ -157,13 +163,13 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
     } else {
 #pragma unroll
       for (int q = 0; q < RISM_MAX_HIST; ++q)
-        if (q < H.nh) s[1 + q] += r * H.r[q][voff + i];
+        if (q < H.nh) s[1 + q] += r * (double)H.r[q][voff + i];
     }
   };
-  auto load = [&](int) -> cplx { return mk(0.0, 0.0); };
-  auto store = [&](int p, cplx zv) {
+  auto load = [&](int) -> C { return C{0, 0}; };
+  auto store = [&](int p, C zv) {
     if (!ok) return;
-    const cplx zz = scale * cmulc(zv, zpre[p]);
+    const C zz = scale * cmulc(zv, zpre[p]);
     resid(2 * p, zz.x);
     resid(2 * p + 1, zz.y);
   };
@@ -204,22 +210,22 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
 static inline unsigned zgrid(const Params &P, int B) {
   return (unsigned)(((P.nx + B - 1) / B) * P.ny * P.natv);
 }
-template <int M>
-static void launch_zfwd(const Params &P, const double *t, const double *u, const double *fr,
-                        cplx *spec, const cplx *twM, const cplx *zpre, const cplx *zpost,
+template <int M, class Re>
+static void launch_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<Re> *spec,
+                        const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost,
                         hipStream_t st) {
-  constexpr int B = tile_cols<M>();
-  hipLaunchKernelGGL((k_zfwd_closure<M, B>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0, st,
-                     t, u, fr, P, spec, twM, zpre, zpost);
+  constexpr int B = tile_cols<M, Re>();
+  hipLaunchKernelGGL((k_zfwd_closure<M, B, Re>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0,
+                     st, t, u, fr, P, spec, twM, zpre, zpost);
 }
-template <int M, int HM>
-static void launch_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
-                        double *R, const double *R0, const double *R1, const Hist &H,
-                        double *partial, const cplx *twM, const cplx *zpre, const cplx *zpost,
-                        int use_hist, hipStream_t st) {
-  constexpr int B = tile_cols<M>();
-  hipLaunchKernelGGL((k_zinv_resid<M, B, HM>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0,
-                     st, spec, t, u, R, R0, R1, H, P, partial, twM, zpre, zpost, use_hist);
+template <int M, int HM, class Re>
+static void launch_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
+                        const Re *R0, const Re *R1, const HistT<Re> &H, double *partial,
+                        const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost, int use_hist,
+                        hipStream_t st) {
+  constexpr int B = tile_cols<M, Re>();
+  hipLaunchKernelGGL((k_zinv_resid<M, B, HM, Re>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T),
+                     0, st, spec, t, u, R, R0, R1, H, P, partial, twM, zpre, zpost, use_hist);
 }
 bool fft_length_supported(int L) {
   switch (L) {
@@ -231,13 +237,16 @@ bool fft_length_supported(int L) {
       return false;
   }
 }
+// Workgroups of one z pass; the same for both bulk types (the FP32 tiles are
+// twice as wide, so pass the type the solver runs in).
+template <class Re>
 size_t zinv_num_blocks(const Params &P) {
   const int M = P.nz / 2;
   int B = 16;
   switch (M) {
-#define X(n) \
-  case n:    \
-    B = tile_cols<n>(); \
+#define X(n)               \
+  case n:                  \
+    B = tile_cols<n, Re>(); \
     break;
     RISM_LENGTHS(X)
 #undef X
@@ -246,9 +255,9 @@ size_t zinv_num_blocks(const Params &P) {
   }
   return (size_t)((P.nx + B - 1) / B) * P.ny * P.natv;
 }
-hipError_t run_zfwd(const Params &P, const double *t, const double *u, const double *fr,
-                    cplx *spec, const cplx *twM, const cplx *zpre, const cplx *zpost,
-                    hipStream_t st) {
+template <class Re>
+hipError_t run_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<Re> *spec,
+                    const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost, hipStream_t st) {
   switch (P.nz / 2) {
 #define X(n)                                                 \
   case n:                                                    \
@@ -261,14 +270,15 @@ hipError_t run_zfwd(const Params &P, const double *t, const double *u, const dou
   }
   return hipGetLastError();
 }
-hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const double *u,
-                    double *R, const double *R0, const double *R1, double *partial,
-                    const cplx *twM, const cplx *zpre, const cplx *zpost, int use_hist,
-                    hipStream_t st) {
+template <class Re>
+hipError_t run_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
+                    const Re *R0, const Re *R1, double *partial, const cx<Re> *twM,
+                    const cx<Re> *zpre, const cx<Re> *zpost, int use_hist, hipStream_t st) {
   switch (P.nz / 2) {
-#define X(n)                                                                             \
-  case n:                                                                                \
-    launch_zinv<n, 0>(P, spec, t, u, R, R0, R1, Hist{}, partial, twM, zpre, zpost, use_hist, st); \
+#define X(n)                                                                                   \
+  case n:                                                                                      \
+    launch_zinv<n, 0>(P, spec, t, u, R, R0, R1, HistT<Re>{}, partial, twM, zpre, zpost,       \
+                      use_hist, st);                                                           \
     break;
     RISM_LENGTHS(X)
 #undef X
@@ -278,13 +288,15 @@ hipError_t run_zinv(const Params &P, const cplx *spec, const double *t, const do
   return hipGetLastError();
 }
 
-hipError_t run_zinv_hist(const Params &P, const cplx *spec, const double *t, const double *u,
-                         double *R, const Hist &H, double *partial, const cplx *twM,
-                         const cplx *zpre, const cplx *zpost, hipStream_t st) {
+template <class Re>
+hipError_t run_zinv_hist(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
+                         const HistT<Re> &H, double *partial, const cx<Re> *twM,
+                         const cx<Re> *zpre, const cx<Re> *zpost, hipStream_t st) {
   switch (P.nz / 2) {
-#define X(n)                                                                                    \
-  case n:                                                                                       \
-    launch_zinv<n, 1>(P, spec, t, u, R, nullptr, nullptr, H, partial, twM, zpre, zpost, 0, st); \
+#define X(n)                                                                                 \
+  case n:                                                                                    \
+    launch_zinv<n, 1>(P, spec, t, u, R, (const Re *)nullptr, (const Re *)nullptr, H, partial, \
+                      twM, zpre, zpost, 0, st);                                              \
     break;
     RISM_LENGTHS(X)
 #undef X
@@ -293,5 +305,21 @@ hipError_t run_zinv_hist(const Params &P, const cplx *spec, const double *t, con
   }
   return hipGetLastError();
 }
+
+#define INST(Re)                                                                              \
+  template size_t zinv_num_blocks<Re>(const Params &);                                       \
+  template hipError_t run_zfwd<Re>(const Params &, const Re *, const Re *, const Re *,       \
+                                   cx<Re> *, const cx<Re> *, const cx<Re> *, const cx<Re> *, \
+                                   hipStream_t);                                             \
+  template hipError_t run_zinv<Re>(const Params &, const cx<Re> *, const Re *, const Re *,   \
+                                   Re *, const Re *, const Re *, double *, const cx<Re> *,   \
+                                   const cx<Re> *, const cx<Re> *, int, hipStream_t);        \
+  template hipError_t run_zinv_hist<Re>(const Params &, const cx<Re> *, const Re *,          \
+                                        const Re *, Re *, const HistT<Re> &, double *,       \
+                                        const cx<Re> *, const cx<Re> *, const cx<Re> *,      \
+                                        hipStream_t);
+INST(double)
+INST(float)
+#undef INST
 
 }  // namespace rism

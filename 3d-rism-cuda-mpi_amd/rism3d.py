@@ -24,6 +24,12 @@ RISM_ERR_ARG, RISM_ERR_IO, RISM_ERR_VERSION, RISM_ERR_CLOSURE = 1, 2, 3, 4
 RISM_ERR_NATOM, RISM_ERR_KRANGE, RISM_ERR_HIP, RISM_ERR_NOMEM, RISM_ERR_STATE = 5, 6, 7, 8, 9
 STATUS_CONVERGED, STATUS_DIVERGED, STATUS_MAXSTEP = 0, 1, 2
 KH, HNC = 0, 1
+
+
+def PSE(n):
+    """Closure code of PSE-n (Kast-Kloss partial series, 1 <= n <= 20)."""
+    return 10 + int(n)
+
 FIELD_T, FIELD_G, FIELD_H, FIELD_R, FIELD_U, FIELD_FR, FIELD_C = range(7)
 
 # every symbol include/rism3d_c.h declares
@@ -34,7 +40,7 @@ EXPORTS = [
     "rism_read_input", "rism_free_input", "rism_read_xvk", "rism_write_xmu",
     "rism_write_guv", "rism_write_tuv", "rism_read_tuv", "rism_k_fft_r2c",
     "rism_k_fft_c2r", "rism_k_get_fk", "rism_k_an2_state", "rism_set_kernel_timing",
-    "rism_get_kernel_stats",
+    "rism_get_kernel_stats", "rism_set_precision", "rism_get_precision",
 ]
 KERNELS = ["zfwd_closure", "ypass_fwd", "xpass_oz", "ypass_inv", "zinv_resid", "reduce",
            "an2_update", "potential_r", "fk_half", "thermo_partial"]
@@ -69,7 +75,8 @@ class Opts(ctypes.Structure):
                 ("an2_m", ctypes.c_double), ("an2_mp", ctypes.c_double),
                 ("ad_switch", ctypes.c_int), ("lambda_", ctypes.c_double),
                 ("verbose", ctypes.c_int), ("chi_spline", ctypes.c_int),
-                ("anderson_m", ctypes.c_int)]
+                ("anderson_m", ctypes.c_int), ("precision", ctypes.c_int),
+                ("mixed_switch", ctypes.c_double)]
 
 
 class Thermo(ctypes.Structure):
@@ -93,7 +100,8 @@ class Thermo(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [("setup_ms", ctypes.c_double), ("iterate_ms", ctypes.c_double),
                 ("thermo_ms", ctypes.c_double), ("steps", ctypes.c_int),
-                ("status", ctypes.c_int), ("rms", ctypes.c_double)]
+                ("status", ctypes.c_int), ("rms", ctypes.c_double),
+                ("steps_fp32", ctypes.c_int)]
 
 
 class KStat(ctypes.Structure):
@@ -137,6 +145,8 @@ def lib():
     L.rism_setup_potentials.argtypes = [vp]
     L.rism_calculate.argtypes = [vp, _dp]
     L.rism_anderson.argtypes = [vp]
+    L.rism_set_precision.argtypes = [vp, ci]
+    L.rism_get_precision.argtypes = [vp]
     L.rism_solve.argtypes = [vp, ci, cd, pi, pi, _dp]
     L.rism_thermo.argtypes = [vp, ctypes.POINTER(Thermo)]
     L.rism_get_field.argtypes = [vp, ci, ci, _dp]
@@ -267,7 +277,8 @@ class RISM3D:
     """
 
     def __init__(self, grid, box, solvent, solute, closure=KH, count=2, m=0.6, mp=0.2,
-                 ad_switch=0, lam=1.0, verbose=False, chi_spline=False, anderson_m=0):
+                 ad_switch=0, lam=1.0, verbose=False, chi_spline=False, anderson_m=0,
+                 precision=0, mixed_switch=0.0):
         self.grid = [int(g) for g in grid]
         self.box = [float(b) for b in box]
         self.natv = int(solvent["natv"])
@@ -286,7 +297,8 @@ class RISM3D:
         su = SoluteC(natu, *[_ptr(a) for a in us])
         g = Grid(self.grid[0], self.grid[1], self.grid[2], (ctypes.c_double * 3)(*self.box))
         op = Opts(int(closure), int(count), float(m), float(mp), int(ad_switch), float(lam),
-                  int(bool(verbose)), int(bool(chi_spline)), int(anderson_m))
+                  int(bool(verbose)), int(bool(chi_spline)), int(anderson_m), int(precision),
+                  float(mixed_switch))
         err = ctypes.c_int(0)
         self._ctx = lib().rism_create(ctypes.byref(g), ctypes.byref(sv), ctypes.byref(su),
                                       ctypes.byref(op), ctypes.byref(err))
@@ -296,10 +308,11 @@ class RISM3D:
 
     @classmethod
     def from_inp(cls, control, structure=None, center=True, q0=False, solvent_dir=None,
-                 ad_switch=0, lam=1.0, verbose=False, chi_spline=False, anderson_m=0):
+                 ad_switch=0, lam=1.0, verbose=False, chi_spline=False, anderson_m=0,
+                 precision=0, mixed_switch=0.0):
         d = read_input(control, structure, center, q0, solvent_dir)
         obj = cls(d["grid"], d["box"], d, d, d["closure"], d["count"], d["m"], d["mp"],
-                  ad_switch, lam, verbose, chi_spline, anderson_m)
+                  ad_switch, lam, verbose, chi_spline, anderson_m, precision, mixed_switch)
         obj.input = d
         obj.shift = d["shift"]
         return obj
@@ -359,7 +372,15 @@ class RISM3D:
         self.last_thermo = th
         return th.as_dict(self.closure), {
             "setup_ms": tm.setup_ms, "iterate_ms": tm.iterate_ms, "thermo_ms": tm.thermo_ms,
-            "steps": tm.steps, "status": tm.status, "rms": tm.rms}
+            "steps": tm.steps, "status": tm.status, "rms": tm.rms,
+            "steps_fp32": tm.steps_fp32}
+
+    def set_precision(self, prec):
+        """Moves the loop state to FP64 (0) or FP32 (1) (mixed contexts)."""
+        _check(lib().rism_set_precision(self._ctx, int(prec)))
+
+    def precision(self):
+        return lib().rism_get_precision(self._ctx)
 
     def get(self, which, site=0):
         out = np.zeros(self.N)

@@ -166,6 +166,9 @@ def main():
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--anderson-m", type=int, default=None,
                     help="history length of Anderson(m) mixing (0: the reference's AN2)")
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
+                    help="fp64: the reference's precision; mixed: FP32 bulk + FP64 sums and "
+                         "final FP64 steps; both: time both (value = FP64)")
     ap.add_argument("--beta", type=float, default=0.6,
                     help="mixing coefficient (AN2's m; beta of Anderson(m))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -214,6 +217,7 @@ def main():
     wall = time.perf_counter() - t0
     ks = g.kernel_stats()
     g.kernel_timing(False)
+    g.close()
     th, tm = results[-1]
     if tm["status"] != rism.STATUS_CONVERGED:
         log(f"[bench] WARNING: solve did not converge (status {tm['status']})")
@@ -235,6 +239,39 @@ def main():
     iter_bytes = sum(ab.values())
     iter_ms = tm["iterate_ms"] / max(iters, 1)
     log("[bench] per-kernel:", json.dumps(per_kernel, indent=None))
+
+    mixed = None
+    if args.precision in ("mixed", "both"):
+        gm = rism.RISM3D(grid, box, solv, solu, rism.KH, 2, args.beta, 0.2, anderson_m=am,
+                         precision=1)
+        for i in range(args.warmup):
+            thm, tmm = gm.run(maxstep=args.maxstep, tol=1e-6)
+            log(f"[bench] mixed warmup {i}: {tmm}")
+        gm.kernel_timing(True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(args.steps):
+            thm, tmm = gm.run(maxstep=args.maxstep, tol=1e-6)
+            log(f"[bench] mixed step {i}: {tmm} SFE_SC {thm['SFE_SC']:.5f}")
+        torch.cuda.synchronize()
+        wall_m = time.perf_counter() - t1
+        ksm = gm.kernel_stats()
+        gm.kernel_timing(False)
+        gm.close()
+        dmu = thm["SFE_SC"] - th["SFE_SC"]
+        mixed = {
+            "s_per_solve": wall_m / args.steps,
+            "speedup_vs_fp64": (ms_per_step / 1000.0) / (wall_m / args.steps),
+            "iterations": tmm["steps"], "iterations_fp32": tmm["steps_fp32"],
+            "status": tmm["status"], "setup_ms": tmm["setup_ms"],
+            "iterate_ms": tmm["iterate_ms"],
+            "SFE_SC": thm["SFE_SC"], "dSFE_SC_J_per_mol": dmu,
+            "dSFE_SC_kcal_per_mol": dmu / 4184.0,
+            "gate_0.05_kcal_per_mol": abs(dmu) <= 0.05 * 4184.0,
+            "dPMV": thm["PMV"] - th["PMV"],
+            "kernels_avg_ms": {k: v[1] / max(v[0], 1) for k, v in ksm.items() if v[0] > 0},
+        }
+        log(f"[bench] mixed: {mixed}")
 
     cpu = None
     if not args.no_cpu:
@@ -278,6 +315,7 @@ def main():
             "avg_launch_ms": avg_s * 1000.0,
         },
         "cpu_baseline": cpu,
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
 

@@ -45,6 +45,9 @@ extern "C" {
 /* ---- closures (read_input.cu:38-47; PSE-n is an extension) -------------- */
 #define RISM_CLOSURE_KH 0
 #define RISM_CLOSURE_HNC 1
+/* Kast-Kloss partial series expansion of order n, 1 <= n <= 20 (.inp keyword
+ * PSEn or PSE-n); PSE-1 equals KH and PSE-n tends to HNC as n grows. */
+#define RISM_CLOSURE_PSE(n) (10 + (n))
 
 /* ---- fields for rism_get_field / rism_set_field ------------------------ */
 #define RISM_FIELD_T 0   /* t_uv (r-space), [natv][nz][ny][nx]                */
@@ -87,7 +90,7 @@ typedef struct {
 /* Options: closure, AN2 parameters (.inp line 5, anderson.h:18-20) and the
  * -l / -e lambda switches (main.cu:27-32, set_ad.cu). */
 typedef struct {
-  int closure;           /* RISM_CLOSURE_KH / RISM_CLOSURE_HNC               */
+  int closure;           /* RISM_CLOSURE_KH / _HNC / _PSE(n)                 */
   int an2_count;         /* Picard warm-up steps                             */
   double an2_m;          /* mixing coefficient                               */
   double an2_mp;         /* extra weight on the newest history entry         */
@@ -101,6 +104,12 @@ typedef struct {
                          /* 1..16: Anderson mixing over that many history    */
                          /* pairs, beta = an2_m, after an2_count Picard      */
                          /* steps (no counterpart in the reference)          */
+  int precision;         /* 0: FP64 throughout (the reference); 1: mixed --   */
+                         /* FFTs, closure, OZ and the accelerator's fields in */
+                         /* FP32 with FP64 sums, then FP64 steps to tol       */
+  double mixed_switch;   /* rms at which the mixed solve moves to FP64        */
+                         /* (<= tol: at tol); it also moves when the FP32 rms */
+                         /* stops improving                                   */
 } rism_opts;
 
 /* Thermodynamics of the converged state (output.cu:23-46, output_xmu.cu). */
@@ -125,6 +134,7 @@ typedef struct {
   int steps;
   int status;
   double rms;
+  int steps_fp32;           /* steps of a mixed solve taken at FP32         */
 } rism_timing;
 
 typedef struct rism_ctx rism_ctx;
@@ -159,6 +169,13 @@ int rism_calculate(rism_ctx *ctx, double *rms);
 /* Replaces AN2::calculate (anderson.cu:43-78): Picard warm-up or the
  * two-slot modified Anderson update of t from the last residual. */
 int rism_anderson(rism_ctx *ctx);
+/* Mixed-precision contexts (opts.precision = 1; no counterpart in the
+ * reference): moves the loop state between the FP64 level (prec 0) and the
+ * FP32 level (prec 1), keeping the accelerator's history.  rism_solve moves
+ * to FP64 by itself; rism_get_precision returns the level the next
+ * rism_calculate runs at. */
+int rism_set_precision(rism_ctx *ctx, int prec);
+int rism_get_precision(rism_ctx *ctx);
 /* Replaces the loop of RISM3D::iterate (iterate.cu:48-82): calculate, rms,
  * divergence and convergence tests, AN2 update; at most maxstep steps.
  * *status is a RISM_STATUS_* value, *steps the steps taken, *rms the last

@@ -227,8 +227,7 @@ def read_inp(control_path, structure_path=None, center=True, q0=False):
     if check != VERSION:
         raise ValueError("Error: This input file is for old version.")
     closure = nxt()
-    if closure not in ("KH", "HNC"):
-        raise ValueError("Error: Unexpected closure switch ")
+    closure_code(closure)
     fsolvent = nxt()
     convergence, maxstep = float(nxt()), int(nxt())
     count, m, mp = int(nxt()), float(nxt()), float(nxt())
@@ -330,7 +329,7 @@ class OracleRISM:
         self.box = np.array(box, dtype=float)
         self.solv = solv
         self.sol = sol
-        self.clos = {"KH": 0, "HNC": 1}[closure]
+        self.clos = closure_code(closure)
         self.count, self.m, self.mp = int(count), float(m), float(mp)
         self.tol, self.maxstep = float(tol), int(maxstep)
         self.lam, self.adswitch = lam, adswitch
@@ -406,7 +405,18 @@ class OracleRISM:
         r = self.result if h is None else None
         h = r.h if h is None else h
         t = r.t if t is None else t
-        return thermo(self.solv, self.dv, self.clos, h, t)
+        return thermo(self.solv, self.dv, self.clos, h, t, self.u)
+
+
+def closure_code(name):
+    """KH -> 0, HNC -> 1 (read_input.cu:38-47); PSEn / PSE-n -> 10 + n."""
+    if name in ("KH", "HNC"):
+        return {"KH": 0, "HNC": 1}[name]
+    if name.startswith("PSE"):
+        num = name[4:] if name[3:4] == "-" else name[3:]
+        if num.isdigit() and 1 <= int(num) <= 20:
+            return 10 + int(num)
+    raise ValueError("Error: Unexpected closure switch ")
 
 
 class AndersonM:
@@ -452,16 +462,25 @@ class AndersonM:
         return tn
 
 
-def thermo(solv: Solvent, dv, clos, h, t):
-    """cal_exchem/cal_pmv/cal_pressure -> dict of .xmu quantities (J/mol etc.)."""
+def thermo(solv: Solvent, dv, clos, h, t, u=None):
+    """cal_exchem/cal_pmv/cal_pressure -> dict of .xmu quantities (J/mol etc.).
+
+    PSE-n (clos = 10 + n, an extension): the SC integrand is the HNC one minus
+    (t*)^(n+1)/(n+1)! where t* = -u + t > 0 (Kast & Kloss 2008); needs u."""
     L = lib()
     n = solv.natv
     N = len(h) // n
     xmu = np.zeros(2 * n)
     xmu2 = np.zeros(n)
     cuv = ctypes.c_double(0.0)
-    L.or_thermo(n, N, float(dv), int(clos), np.ascontiguousarray(solv.rho),
+    L.or_thermo(n, N, float(dv), int(clos if clos < 10 else 1),
+                np.ascontiguousarray(solv.rho),
                 np.ascontiguousarray(h), np.ascontiguousarray(t), xmu, xmu2, ctypes.byref(cuv))
+    if clos >= 10:
+        order = clos - 10
+        e = (np.asarray(t) - np.asarray(u)).reshape(n, N)
+        corr = np.where(e > 0, np.maximum(e, 0) ** (order + 1) / math.factorial(order + 1), 0.0)
+        xmu[:n] -= corr.sum(axis=1) * solv.rho * dv
     pmv = L.or_pmv(solv.xt, solv.temper, cuv.value)
     pres = L.or_pressure(n, np.ascontiguousarray(solv.rho), solv.xt, solv.temper)
     ibeta = AVOGADRO * BOLTZMANN * solv.temper

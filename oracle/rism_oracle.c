@@ -457,9 +457,20 @@ void or_fk(int nx, int ny, int nz, double bx, double by, double bz, int natu,
 /* ------------------------------------------------------------------------- */
 
 /* closure: clos 0 = KH, 1 = HNC (calculate_cuda.cu:1-16) */
+/* clos 0 KH, 1 HNC (calculate_cuda.cu:1-16); 10 + n: PSE-n (Kast & Kloss,
+ * J. Chem. Phys. 129, 214110 (2008)), the partial sum of exp's series
+ * sum_{i=0}^{n} e^i / i! where e > 0 (an extension; not in the reference) */
 static inline double closure_g(int clos, double t, double u) {
   double earg = -u + t;
   if (clos == 0 && earg >= 0.0) return 1.0 + earg;
+  if (clos >= 10 && earg > 0.0) {
+    double term = 1.0, sum = 1.0;
+    for (int i = 1; i <= clos - 10; ++i) {
+      term *= earg / (double)i;
+      sum += term;
+    }
+    return sum;
+  }
   return exp(earg);
 }
 

@@ -58,12 +58,12 @@ def test_potentials(rism, oracle, gpu, case24):
     g.close()
 
 
-@pytest.mark.parametrize("closure", ["KH", "HNC"])
+@pytest.mark.parametrize("closure", ["KH", "HNC", "PSE3"])
 def test_one_step(rism, oracle, gpu, case24, closure):
     sol, s, d, grid, box = case24
     o = oracle.OracleRISM(grid, box, s, sol, closure).setup()
     rms_o, g_o, R_o, h_o = o.calculate(o.t0)
-    g = rism.RISM3D(grid, box, d, _solute(sol), closure={"KH": 0, "HNC": 1}[closure])
+    g = rism.RISM3D(grid, box, d, _solute(sol), closure=oracle.closure_code(closure))
     g.initialize()
     rms_g = g.calculate()
     assert abs(rms_g - rms_o) < 1e-11 * rms_o
@@ -92,16 +92,18 @@ def test_an2_trajectory(rism, oracle, gpu, case24):
     g.close()
 
 
-@pytest.mark.parametrize("closure,qscale", [("KH", 1.0), ("KH", 0.0), ("HNC", 0.0)])
+@pytest.mark.parametrize("closure,qscale", [("KH", 1.0), ("KH", 0.0), ("HNC", 0.0),
+                                           ("PSE3", 0.0), ("PSE2", 0.3)])
 def test_full_solve_vs_oracle(rism, oracle, gpu, case24, closure, qscale):
     """KH with the full charges; HNC (whose exp(-u + t) overflows for this
-    strongly charged solute, see test_divergence) with scaled charges."""
+    strongly charged solute, see test_divergence) and PSE-n (whose series
+    also runs away in the first steps at full charge) with scaled charges."""
     sol, s, d, grid, box = case24
     sol = oracle.Solute(sol.sig, sol.eps, sol.q * qscale, sol.xyz)
     o = oracle.OracleRISM(grid, box, s, sol, closure, tol=1e-6, maxstep=400).setup()
     ro = o.solve()
     to = o.thermo()
-    g = rism.RISM3D(grid, box, d, _solute(sol), closure={"KH": 0, "HNC": 1}[closure])
+    g = rism.RISM3D(grid, box, d, _solute(sol), closure=oracle.closure_code(closure))
     g.initialize()
     st, n, rms = g.iterate(maxstep=400, tol=1e-6)
     print(closure, qscale, "steps", n, ro.steps)

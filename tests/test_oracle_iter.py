@@ -263,3 +263,57 @@ def test_anderson_m_two_matches_an2_least_squares(oracle):
     x = np.zeros(2)
     oracle.lib().or_leg2(G, c, x)
     assert np.allclose(acc.coefficients(R2), x, rtol=1e-10, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# PSE-n closure (an extension): pinned by PSE-1 == KH, the series closed form
+# and the HNC limit.
+# ---------------------------------------------------------------------------
+def _closure(oracle, clos, t, u):
+    g = np.zeros_like(t)
+    oracle.lib().or_closure(clos, len(t), np.ascontiguousarray(t), np.ascontiguousarray(u), g)
+    return g
+
+
+def test_pse_closure(oracle):
+    rng = np.random.default_rng(21)
+    t = rng.normal(0, 2, 4000)
+    u = rng.normal(0, 2, 4000)
+    e = -u + t
+    kh, hnc = _closure(oracle, 0, t, u), _closure(oracle, 1, t, u)
+    assert np.array_equal(_closure(oracle, oracle.closure_code("PSE1"), t, u), kh)
+    g3 = _closure(oracle, oracle.closure_code("PSE-3"), t, u)
+    want = np.where(e > 0, 1 + e + e**2 / 2 + e**3 / 6, np.exp(e))
+    assert np.allclose(g3, want, rtol=1e-14, atol=0)
+    g20 = _closure(oracle, oracle.closure_code("PSE20"), t, u)
+    # Taylor remainder: 0 <= exp(e) - PSE20 <= exp(e) e^21 / 21! for e > 0
+    import math
+    rem = np.where(e > 0, np.exp(e) * np.maximum(e, 0) ** 21 / math.factorial(21), 0.0)
+    assert np.all(hnc - g20 >= -1e-14 * hnc)  # up to the rounding of 21 terms
+    assert np.all(hnc - g20 <= rem + 1e-14 * hnc)
+    # between KH and HNC, increasing with n where e > 0
+    g2 = _closure(oracle, 12, t, u)
+    pos = e > 0
+    assert np.all(kh[pos] <= g2[pos]) and np.all(g2[pos] <= g3[pos]) and np.all(g3[pos] <= hnc[pos])
+    with pytest.raises(ValueError):
+        oracle.closure_code("PSE0")
+
+
+def test_pse_thermo_reduces(oracle):
+    """SC integrand of PSE-n: HNC's minus (t*)^(n+1)/(n+1)! for t* > 0; with
+    t* <= 0 everywhere it equals HNC's; PSE-1 equals KH's where h = t*."""
+    rng = np.random.default_rng(22)
+    natv, N = 2, 500
+    solv = oracle.Solvent(natv, 1, 0.1, 300.0, 4.5e-10, np.ones(natv), np.ones(natv),
+                          np.zeros(natv), np.array([0.03, 0.01]), np.zeros((1, natv, natv)))
+    t = rng.normal(0, 1, natv * N)
+    u = t + np.abs(rng.normal(0, 1, natv * N))  # t* = -u + t <= 0
+    h = np.exp(-u + t) - 1.0
+    a = oracle.thermo(solv, 0.1, 13, h, t, u)
+    b = oracle.thermo(solv, 0.1, 1, h, t)
+    assert a["SFE_SC"] == pytest.approx(b["SFE_SC"], rel=1e-14)
+    u2 = t - np.abs(rng.normal(0, 1, natv * N))  # t* > 0, KH: h = t*
+    h2 = -u2 + t
+    kh = oracle.thermo(solv, 0.1, 0, h2, t)
+    p1 = oracle.thermo(solv, 0.1, 11, h2, t, u2)
+    assert p1["SFE_SC"] == pytest.approx(kh["SFE_SC"], rel=1e-12)
