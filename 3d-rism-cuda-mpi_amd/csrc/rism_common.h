@@ -26,6 +26,33 @@ __device__ __forceinline__ Re closure_g(int clos, Re t, Re u) {
   return s;
 }
 
+__device__ __forceinline__ double rexpm1(double a) { return expm1(a); }
+__device__ __forceinline__ float rexpm1(float a) { return expm1f(a); }
+
+// g(a + d) - g(a) for e = a -> a + d, evaluated without cancellation when
+// |d| << |a| (the difference form of a correction sweep): exp(a) expm1(d)
+// on the exponential branch, d on KH's linear one, d times the divided
+// difference of the PSE-n polynomial on its series branch; across a branch
+// point |a|, |a + d| <= |d| and the plain difference is exact enough.
+template <class Re>
+__device__ __forceinline__ Re closure_dg(int clos, Re a, Re d) {
+  const Re b = a + d;
+  if (clos == 1 || (a < (Re)0 && b < (Re)0)) return rexp(a) * rexpm1(d);
+  if (a >= (Re)0 && b >= (Re)0) {
+    if (clos == 0) return d;
+    // PSE-n: P[a, b] = sum_{i>=1} h_{i-1}(a, b) / i!, h_k = sum_j a^j b^(k-j)
+    Re pa = (Re)1, h = (Re)1, fact = (Re)1, s = (Re)1;
+    for (int i = 2; i <= clos - 10; ++i) {
+      pa *= a;
+      h = b * h + pa;
+      fact *= (Re)i;
+      s += h / fact;
+    }
+    return d * s;
+  }
+  return closure_g(clos, b, (Re)0) - closure_g(clos, a, (Re)0);
+}
+
 // the PSE-n correction to the HNC integrand of the excess chemical
 // potential: e^(n+1) / (n+1)! where e = -u + t > 0
 __device__ __forceinline__ double pse_excess(int n, double e) {

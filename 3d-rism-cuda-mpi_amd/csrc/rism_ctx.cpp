@@ -98,6 +98,7 @@ struct Level {
   cx<Re> *fk = nullptr, *spec = nullptr;
   std::vector<Re *> Rb, Pb;  // Anderson(m) residuals and mixed iterates
   Re *Rlast = nullptr;       // residual written by the last step at this level
+  Re *aux = nullptr;         // FP32 level: the FP64 residual r0 of a correction sweep
   TablesT<Re> tb;
 };
 
@@ -596,12 +597,15 @@ int an2_step(rism_ctx *c) {
 }
 
 // One calculate() + cal_rms() at level Re (calculate.cu:4-65, cal_rms.cu:14-26)
+// lin: the difference-form step of a correction sweep (t = the correction
+// d, u = the closure exponent at t0, aux = the residual at t0).
 template <class Re>
-int calc_step(rism_ctx *c, double *rms) {
+int calc_step(rism_ctx *c, double *rms, bool lin = false) {
   Level<Re> &L = lv<Re>(c);
   rism::Params P = c->P;
-  P.mode = 0;
+  P.mode = lin ? 3 : 0;
   Re *t = L.T[c->iT_cur];
+  const Re *r0 = lin ? L.aux : nullptr;
   {
     KTime kt(c, RISM_K_ZFWD);
     HIPCHECK(rism::run_zfwd(P, t, L.u, L.fr, L.spec, L.tb.twz, L.tb.zpre, L.tb.zpost, c->st));
@@ -628,12 +632,12 @@ int calc_step(rism_ctx *c, double *rms) {
       rism::HistT<Re> H{};
       H.nh = (int)c->hist.size();
       for (int p = 0; p < H.nh; ++p) H.r[p] = L.Rb[c->hist[p].rb];
-      HIPCHECK(rism::run_zinv_hist(P, L.spec, t, L.u, Rout, H, c->partial, L.tb.twz, L.tb.zpre,
-                                   L.tb.zpost, c->st));
+      HIPCHECK(rism::run_zinv_hist(P, L.spec, t, L.u, Rout, r0, H, c->partial, L.tb.twz,
+                                   L.tb.zpre, L.tb.zpost, c->st));
     } else {
       const int use_hist = (c->count == 0);
       HIPCHECK(rism::run_zinv(P, L.spec, t, L.u, Rout, L.R[c->iR_slot[0]], L.R[c->iR_slot[1]],
-                              c->partial, L.tb.twz, L.tb.zpre, L.tb.zpost, use_hist, c->st));
+                              r0, c->partial, L.tb.twz, L.tb.zpre, L.tb.zpost, use_hist, c->st));
     }
   }
   const size_t nb = rism::zinv_num_blocks<Re>(c->P) / c->natv;
@@ -704,6 +708,137 @@ int convert_state(rism_ctx *c) {
   return RISM_OK;
 }
 
+// Restarts the accelerator at level Re: the reference's initial AN2 state
+// (Picard warm-up, zeroed slots) or an empty Anderson(m) history.
+template <class Re>
+int restart_accel(rism_ctx *c) {
+  Level<Re> &L = lv<Re>(c);
+  const size_t nn = c->N * c->natv;
+  c->count = c->opts.an2_count;
+  c->binary = 0;
+  c->iR_cur = 0;
+  c->iR_slot[0] = 1;
+  c->iR_slot[1] = 2;
+  c->hist.clear();
+  c->cur_rb = 0;
+  c->next_pb = 0;
+  if (c->opts.anderson_m == 0) {
+    for (int i = 0; i < 2; ++i) {
+      HIPCHECK(hipMemsetAsync(L.Pslot[i], 0, sizeof(Re) * nn, c->st));
+      HIPCHECK(hipMemsetAsync(L.R[c->iR_slot[i]], 0, sizeof(Re) * nn, c->st));
+    }
+  }
+  return RISM_OK;
+}
+
+template <class Re>
+int accel_step(rism_ctx *c) {
+  c->fresh = false;
+  return c->opts.anderson_m > 0 ? anderson_m_step<Re>(c) : an2_step<Re>(c);
+}
+
+// The mixed-precision solve (no counterpart in the reference).
+//  1. FP32 steps of the fixed-point map with the accelerator, while the FP32
+//     residual tracks the FP64 one: until rms <= hand_rms, or no progress
+//     for kStall steps.
+//  2. Defect-correction sweeps: one FP64 step at t0 gives the residual r0
+//     (and the convergence test); FP32 steps of the same map written in
+//     difference form, R(t0 + d) = r0 + [h(t0 + d) - h(t0)] -
+//     [g(t0 + d) - g(t0)], then solve for the correction d with the
+//     accelerator, and t0 += d.  The brackets are formed from d and the
+//     base exponent without cancellation, so the FP32 rounding scales with
+//     |d| and |r0|, not with |t|; no linearisation is involved.
+// Leaves the state at the FP64 level.  When a sweep stops gaining (or after
+// kMaxSweeps) it returns with *fallback set, and the caller continues with
+// plain FP64 steps.
+int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, bool &conver,
+                bool &diverge, int &low_steps, bool &fallback) {
+  const bool verbose = c->opts.verbose != 0;
+  const double hand_rms = std::max(tol, c->opts.mixed_switch > 0 ? c->opts.mixed_switch : 1e-3);
+  // a sweep reduces its residual by kEta at most: its FP32 rounding floor
+  // lies a few hundred times below the residual it starts from
+  constexpr int kStall = 25, kMaxSweeps = 12, kMaxInner = 600;
+  constexpr double kEta = 1e-2;
+  const size_t nn = c->N * c->natv;
+  int rc;
+  // 1. FP32 iteration
+  double best = HUGE_VAL;
+  int since_best = 0;
+  while (istep < maxstep) {
+    ++istep;
+    if ((rc = calc_step<float>(c, &rms))) return rc;
+    ++low_steps;
+    if (!std::isfinite(rms)) {
+      diverge = true;
+      return RISM_OK;
+    }
+    if (verbose) std::cout << " Step = " << istep << " Reside = " << rms << " (FP32)" << std::endl;
+    if (rms < 0.99 * best) {
+      best = rms;
+      since_best = 0;
+    } else {
+      ++since_best;
+    }
+    if (rms <= hand_rms || since_best >= kStall) break;
+    if ((rc = accel_step<float>(c))) return rc;
+  }
+  // the t of the last FP32 step becomes t0
+  HIPCHECK(rism::run_convert(c->f.T[c->iT_cur], c->d.T[c->iT_cur], nn, c->st));
+  c->prec = 0;
+  if ((rc = restart_accel<double>(c))) return rc;
+  // 2. correction sweeps
+  double prev = HUGE_VAL;
+  for (int sweep = 0; istep < maxstep; ++sweep) {
+    ++istep;
+    const int k0 = c->iT_cur;
+    if ((rc = calc_step<double>(c, &rms))) return rc;
+    if (!std::isfinite(rms)) {
+      diverge = true;
+      return RISM_OK;
+    }
+    if (verbose) std::cout << " Step = " << istep << " Reside = " << rms << std::endl;
+    if (rms <= tol) {
+      conver = true;
+      return RISM_OK;
+    }
+    if (sweep >= kMaxSweeps || rms > 0.5 * prev) {
+      fallback = true;
+      return RISM_OK;
+    }
+    prev = rms;
+    // the base point t0: its exponent and its residual, in FP32
+    HIPCHECK(rism::run_exponent(c->d.T[k0], c->d.u, nn, c->f.u, c->st));
+    HIPCHECK(rism::run_convert(c->d.Rlast, c->f.aux, nn, c->st));
+    HIPCHECK(hipMemsetAsync(c->f.T[k0], 0, sizeof(float) * nn, c->st));
+    c->prec = 1;
+    if ((rc = restart_accel<float>(c))) return rc;
+    const double target = std::max(kEta * rms, 0.5 * tol);
+    double r = 0.0, rbest = HUGE_VAL;
+    int stall = 0;
+    for (int inner = 0; istep < maxstep; ++inner) {
+      ++istep;
+      if ((rc = calc_step<float>(c, &r, true))) return rc;
+      ++low_steps;
+      if (verbose)
+        std::cout << " Step = " << istep << " Reside = " << r << " (FP32 correction)" << std::endl;
+      if (r < 0.99 * rbest) {
+        rbest = r;
+        stall = 0;
+      } else {
+        ++stall;
+      }
+      if (!std::isfinite(r) || r <= target || stall >= kStall || inner + 1 >= kMaxInner) break;
+      if ((rc = accel_step<float>(c))) return rc;
+    }
+    // t0 += d; the next FP64 step evaluates the corrected t0
+    HIPCHECK(rism::run_add_correction(c->d.T[k0], c->f.T[c->iT_cur], nn, c->st));
+    c->iT_cur = c->iT_last = k0;
+    c->prec = c->last_prec = 0;
+    if ((rc = restart_accel<double>(c))) return rc;
+  }
+  return RISM_OK;
+}
+
 // Device buffers of one level: u, fr, fk, t (ping-pong) and the AN2 or
 // Anderson(m) slots; the FP32 level's spectrum lives in the FP64 one's memory
 // (the two levels never run at the same time).
@@ -714,10 +849,12 @@ int alloc_level(rism_ctx *c, Level<Re> &L, cplx *spec_alias) {
   if ((rc = c->alloc(&L.u, nn))) return rc;
   if ((rc = c->alloc(&L.fr, c->N))) return rc;
   if ((rc = c->alloc(&L.fk, c->N / 2))) return rc;
-  if (spec_alias)
+  if (spec_alias) {  // the FP32 level of a mixed context
     L.spec = reinterpret_cast<cx<Re> *>(spec_alias);
-  else if ((rc = c->alloc(&L.spec, nn / 2)))
+    if ((rc = c->alloc(&L.aux, nn))) return rc;
+  } else if ((rc = c->alloc(&L.spec, nn / 2))) {
     return rc;
+  }
   for (int i = 0; i < 2; ++i)
     if ((rc = c->alloc(&L.T[i], nn))) return rc;
   const int am = c->opts.anderson_m;
@@ -936,38 +1073,23 @@ int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, do
   bool conver = false, diverge = false;
   int istep = 0, last = 0, low_steps = 0;
   double rms = 0.0;
-  // mixed precision: the FP32 level iterates until its rms reaches the
-  // hand-over level (or stops improving), then the state moves to FP64 and
-  // the loop goes on there to tol
-  const double hand_rms = std::max(tol, c->opts.mixed_switch);
-  constexpr int kStall = 25;
-  double best = HUGE_VAL;
-  int since_best = 0;
-  for (istep = 1; istep <= maxstep; ++istep) {  // iterate.cu:48-72
+  if (c->opts.precision == 1 && c->prec == 1) {
+    bool fallback = false;
+    int rc = solve_mixed(c, maxstep, tol, istep, rms, conver, diverge, low_steps, fallback);
+    if (rc) return rc;
+    last = istep;
+    if (fallback) {  // plain FP64 steps from the last FP64 evaluation on
+      if ((rc = restart_accel<double>(c))) return rc;
+      c->fresh = true;
+      if ((rc = accel_step<double>(c))) return rc;
+    }
+  }
+  for (istep = istep + 1; !conver && !diverge && istep <= maxstep; ++istep) {  // iterate.cu:48-72
     int rc = rism_calculate(c, &rms);
     if (rc) return rc;
     last = istep;
     diverge = !std::isfinite(rms);
     if (diverge) break;
-    if (c->prec == 1) {
-      ++low_steps;
-      if (rms < 0.99 * best) {
-        best = rms;
-        since_best = 0;
-      } else {
-        ++since_best;
-      }
-      if (rms <= hand_rms || since_best >= kStall) {
-        if ((rc = rism_set_precision(c, 0))) return rc;
-        if (c->opts.verbose)
-          std::cout << " Step = " << istep << " Reside = " << rms << " (FP32; continuing in FP64)"
-                    << std::endl;
-        continue;  // the next step evaluates the same t in FP64
-      }
-      if ((rc = rism_anderson(c))) return rc;
-      if (c->opts.verbose) std::cout << " Step = " << istep << " Reside = " << rms << std::endl;
-      continue;
-    }
     if (rms <= tol) {
       conver = true;
     } else {
@@ -1302,7 +1424,8 @@ int rism_k_fft_c2r(int nx, int ny, int nz, const double box[3], const double *in
   HIPCHECK(rism::run_apply_phase(spec, P, 1, c->st));
   HIPCHECK(rism::run_xoz(P, spec, spec, c->d.tb.twx, c->d.tb.prex, nullptr, nullptr, nullptr, c->st));
   HIPCHECK(rism::run_y(P, spec, +1, c->d.tb.twy, c->d.tb.prey, c->st));
-  HIPCHECK(rism::run_zinv(P, spec, h, h, h, h, h, partial, c->d.tb.twz, c->d.tb.zpre, c->d.tb.zpost, 0, c->st));
+  HIPCHECK(rism::run_zinv(P, spec, h, h, h, h, h, (const double *)nullptr, partial, c->d.tb.twz,
+                          c->d.tb.zpre, c->d.tb.zpost, 0, c->st));
   HIPCHECK(hipMemcpyAsync(out, h, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->st));
   HIPCHECK(hipStreamSynchronize(c->st));
   return RISM_OK;

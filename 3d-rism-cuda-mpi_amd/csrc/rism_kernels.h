@@ -19,7 +19,8 @@ struct Params {
   int nx, ny, nz;
   int natv;
   int clos;          // 0 KH, 1 HNC
-  int mode;          // 0 solver step; 1 plain forward transform; 2 plain inverse
+  int mode;          // 0 solver step; 1 plain forward transform; 2 plain inverse;
+                     // 3 difference-form step (correction sweep of the mixed solve)
   int shell;         // 1: chi from the cubic-box shell table, 0: spline at |k|
   double dv;         // volume element (forward scale)
   double inv_scale;  // 1 / (dv nx ny (nz/2)): inverse scale of the three passes
@@ -67,15 +68,18 @@ hipError_t run_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const cx<
 hipError_t run_chi_table(const double *knots, const double *coef, const Params &P,
                          double kscale, size_t nshell, double *out, hipStream_t st);
 // AN2 partial sums (6 per workgroup) of the residual pass
+// P.mode 3 (difference form): t = correction d, u = exponent a = -u + t0 at
+// the base point, r0 = the full residual at t0; R = r0 + h - (g(a+d) - g(a)).
 template <class Re>
 hipError_t run_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
-                    const Re *R0, const Re *R1, double *partial, const cx<Re> *twM,
-                    const cx<Re> *zpre, const cx<Re> *zpost, int use_hist, hipStream_t st);
+                    const Re *R0, const Re *R1, const Re *r0, double *partial,
+                    const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost, int use_hist,
+                    hipStream_t st);
 // Anderson(m) variant: partial sums R.R and R.R_p for the nh history vectors
 // (width 1 + RISM_MAX_HIST per workgroup).
 template <class Re>
 hipError_t run_zinv_hist(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
-                         const HistT<Re> &H, double *partial, const cx<Re> *twM,
+                         const Re *r0, const HistT<Re> &H, double *partial, const cx<Re> *twM,
                          const cx<Re> *zpre, const cx<Re> *zpost, hipStream_t st);
 template <class Re>
 hipError_t run_andm_update(Re *tnew, const Re *t, const Re *R, const HistUpdateT<Re> &H, Re *Pb,
@@ -90,6 +94,11 @@ hipError_t run_an2_update(Re *tnew, const Re *t, const Re *R, const Re *P0, cons
                           double s1, double s2, double m, size_t n, hipStream_t st);
 template <class From, class To>
 hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st);
+// closure exponent a = -u + t at the base point t0 of a correction sweep,
+// rounded to FP32
+hipError_t run_exponent(const double *t, const double *u, size_t n, float *out, hipStream_t st);
+// t0 += d (the FP32 correction of the defect-correction step)
+hipError_t run_add_correction(double *t, const float *d, size_t n, hipStream_t st);
 hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
                            double ubeta, int do_coul, double *u, double *fr, hipStream_t st);
 hipError_t run_fk_half(const double *atoms, int natu, int astride, const Params &P,

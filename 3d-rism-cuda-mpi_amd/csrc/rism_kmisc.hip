@@ -119,6 +119,18 @@ __global__ void k_andm_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restr
   Pb[i] = q;
 }
 
+// the closure exponent -u + t, formed in FP64 and rounded once
+__global__ void k_exponent(const double *__restrict__ t, const double *__restrict__ u, size_t n,
+                           float *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)(-u[i] + t[i]);
+}
+
+__global__ void k_add_correction(double *__restrict__ t, const float *__restrict__ d, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] += (double)d[i];
+}
+
 // conversion between the FP64 and FP32 copies of a field
 template <class From, class To>
 __global__ void k_convert(const From *__restrict__ in, To *__restrict__ out, size_t n) {
@@ -458,6 +470,14 @@ hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st) {
   template hipError_t run_thermo_partial<Re>(const Re *, const Re *, const Re *, const Re *,  \
                                              const Params &, size_t, double *, hipStream_t);
 template hipError_t run_convert<double, float>(const double *, float *, size_t, hipStream_t);
+hipError_t run_exponent(const double *t, const double *u, size_t n, float *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_exponent, dim3(nblk(n, 256)), dim3(256), 0, st, t, u, n, out);
+  return hipGetLastError();
+}
+hipError_t run_add_correction(double *t, const float *d, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_correction, dim3(nblk(n, 256)), dim3(256), 0, st, t, d, n);
+  return hipGetLastError();
+}
 template hipError_t run_convert<float, double>(const float *, double *, size_t, hipStream_t);
 
 hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,

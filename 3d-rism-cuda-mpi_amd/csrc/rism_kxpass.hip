@@ -120,7 +120,7 @@ __device__ __forceinline__ void oz_row(cx<Re> *lds, int CS, int jx, int L, const
                                        const Params &P, const Chi &X) {
   const int jm = L - 1 - jx;
   const int off[2] = {lpad(jx), lpad(jm)};
-  const cx<Re> fk[2] = {fkr[jx], fkr[jm]};
+  const cx<Re> fk[2] = {fkr ? fkr[jx] : cx<Re>{0, 0}, fkr ? fkr[jm] : cx<Re>{0, 0}};
   oz_dispatch<NVMAX, 2>(lds, CS, off, fk, P, X);
 }
 
@@ -157,7 +157,8 @@ __global__ void __launch_bounds__(512)
   // OZ at every k of the row, the mirror points (kx, -kx) together
   const int nthr = nv * T;
   const int nv2 = nv * nv;
-  const C *fkr = fkraw + rowoff;
+  // the difference-form step (mode 3): the long-range term cancels, fk = 0
+  const C *fkr = (P.mode == 3) ? nullptr : fkraw + rowoff;
   for (int jx = tid; jx < L / 2; jx += nthr) {
     if constexpr (SHELL) {
       const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates

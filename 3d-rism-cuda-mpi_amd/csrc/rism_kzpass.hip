@@ -47,6 +47,10 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   auto cval = [&](int z) -> Re {
     const size_t i = base + (size_t)z * plane;
     const Re tt = tv[i];
+    // difference form of a correction sweep: t holds the correction d, u
+    // the exponent a = -u + t0 at the base point, and c(t0 + d) - c(t0) =
+    // g(a + d) - g(a) - d (the q fr term cancels)
+    if (P.mode == 3) return (closure_dg(clos, uv[i], tt) - tt) * dv;
     if (P.mode != 0) return tt * dv;  // plain transform of t
     const Re g = closure_g(clos, tt, uv[i]);
     return (g - (Re)1 - tt + q * fr[i]) * dv;  // trm1mt, times the forward dv
@@ -85,7 +89,8 @@ template <int M, int B, int HM, class Re>
 __global__ void __launch_bounds__(B *FftShape<M>::T)
     k_zinv_resid(const cx<Re> *__restrict__ spec, const Re *__restrict__ t,
                  const Re *__restrict__ u, Re *__restrict__ R, const Re *__restrict__ R0,
-                 const Re *__restrict__ R1, HistT<Re> H, Params P, double *__restrict__ partial,
+                 const Re *__restrict__ R1, const Re *__restrict__ r0, HistT<Re> H, Params P,
+                 double *__restrict__ partial,
                  const cx<Re> *__restrict__ twM, const cx<Re> *__restrict__ zpre,
                  const cx<Re> *__restrict__ zpost, int use_hist) {
   using C = cx<Re>;
@@ -137,7 +142,8 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   double s[W];
 #pragma unroll
   for (int k = 0; k < W; ++k) s[k] = 0.0;
-  const int plain = P.mode != 0;
+  const int plain = P.mode == 1 || P.mode == 2;
+  const int lin = P.mode == 3;
   const size_t voff = (size_t)v * N;
   auto resid = [&](int z, Re h) {
     const size_t i = base + (size_t)z * plane;
@@ -145,8 +151,9 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
       Rv[i] = h;
       return;
     }
-    const Re g = closure_g(clos, tv[i], uv[i]);
-    const Re rr = h + (Re)1 - g;  // tr (calculate_cuda.cu:58)
+    // difference form r0 + dh - dg, else tr (calculate_cuda.cu:58)
+    const Re rr = lin ? r0[voff + i] + h - closure_dg(clos, uv[i], tv[i])
+                      : h + (Re)1 - closure_g(clos, tv[i], uv[i]);
     Rv[i] = rr;
     // the sums are FP64 whatever the bulk type
     const double r = (double)rr;
@@ -220,12 +227,12 @@ static void launch_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr,
 }
 template <int M, int HM, class Re>
 static void launch_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
-                        const Re *R0, const Re *R1, const HistT<Re> &H, double *partial,
-                        const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost, int use_hist,
-                        hipStream_t st) {
+                        const Re *R0, const Re *R1, const Re *r0, const HistT<Re> &H,
+                        double *partial, const cx<Re> *twM, const cx<Re> *zpre,
+                        const cx<Re> *zpost, int use_hist, hipStream_t st) {
   constexpr int B = tile_cols<M, Re>();
   hipLaunchKernelGGL((k_zinv_resid<M, B, HM, Re>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T),
-                     0, st, spec, t, u, R, R0, R1, H, P, partial, twM, zpre, zpost, use_hist);
+                     0, st, spec, t, u, R, R0, R1, r0, H, P, partial, twM, zpre, zpost, use_hist);
 }
 bool fft_length_supported(int L) {
   switch (L) {
@@ -272,12 +279,14 @@ hipError_t run_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<
 }
 template <class Re>
 hipError_t run_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
-                    const Re *R0, const Re *R1, double *partial, const cx<Re> *twM,
-                    const cx<Re> *zpre, const cx<Re> *zpost, int use_hist, hipStream_t st) {
+                    const Re *R0, const Re *R1, const Re *r0, double *partial,
+                    const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost, int use_hist,
+                    hipStream_t st) {
+  if (P.mode == 3 && !r0) return hipErrorInvalidValue;
   switch (P.nz / 2) {
 #define X(n)                                                                                   \
   case n:                                                                                      \
-    launch_zinv<n, 0>(P, spec, t, u, R, R0, R1, HistT<Re>{}, partial, twM, zpre, zpost,       \
+    launch_zinv<n, 0>(P, spec, t, u, R, R0, R1, r0, HistT<Re>{}, partial, twM, zpre, zpost,   \
                       use_hist, st);                                                           \
     break;
     RISM_LENGTHS(X)
@@ -290,13 +299,14 @@ hipError_t run_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *
 
 template <class Re>
 hipError_t run_zinv_hist(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
-                         const HistT<Re> &H, double *partial, const cx<Re> *twM,
+                         const Re *r0, const HistT<Re> &H, double *partial, const cx<Re> *twM,
                          const cx<Re> *zpre, const cx<Re> *zpost, hipStream_t st) {
+  if (P.mode == 3 && !r0) return hipErrorInvalidValue;
   switch (P.nz / 2) {
 #define X(n)                                                                                 \
   case n:                                                                                    \
-    launch_zinv<n, 1>(P, spec, t, u, R, (const Re *)nullptr, (const Re *)nullptr, H, partial, \
-                      twM, zpre, zpost, 0, st);                                              \
+    launch_zinv<n, 1>(P, spec, t, u, R, (const Re *)nullptr, (const Re *)nullptr, r0, H,     \
+                      partial, twM, zpre, zpost, 0, st);                                     \
     break;
     RISM_LENGTHS(X)
 #undef X
@@ -312,10 +322,12 @@ hipError_t run_zinv_hist(const Params &P, const cx<Re> *spec, const Re *t, const
                                    cx<Re> *, const cx<Re> *, const cx<Re> *, const cx<Re> *, \
                                    hipStream_t);                                             \
   template hipError_t run_zinv<Re>(const Params &, const cx<Re> *, const Re *, const Re *,   \
-                                   Re *, const Re *, const Re *, double *, const cx<Re> *,   \
-                                   const cx<Re> *, const cx<Re> *, int, hipStream_t);        \
+                                   Re *, const Re *, const Re *, const Re *, double *,       \
+                                   const cx<Re> *, const cx<Re> *, const cx<Re> *, int,      \
+                                   hipStream_t);                                             \
   template hipError_t run_zinv_hist<Re>(const Params &, const cx<Re> *, const Re *,          \
-                                        const Re *, Re *, const HistT<Re> &, double *,       \
+                                        const Re *, Re *, const Re *, const HistT<Re> &,     \
+                                        double *,                                            \
                                         const cx<Re> *, const cx<Re> *, const cx<Re> *,      \
                                         hipStream_t);
 INST(double)
