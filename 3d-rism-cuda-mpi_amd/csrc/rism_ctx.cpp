@@ -99,6 +99,7 @@ struct Level {
   std::vector<Re *> Rb, Pb;  // Anderson(m) residuals and mixed iterates
   Re *Rlast = nullptr;       // residual written by the last step at this level
   Re *aux = nullptr;         // FP32 level: the FP64 residual r0 of a correction sweep
+  Re *chitab = nullptr;      // chi_vv shell table in this level's type
   TablesT<Re> tb;
 };
 
@@ -151,7 +152,7 @@ struct rism_ctx {
   std::vector<double> rho, qv, sigv, epsv;
   double temper = 0, xt = 0;
   int natu = 0;
-  std::vector<double> atoms_host;  // per atom x y z q sig_uv^2[natv] 4eps_uv[natv]
+  std::vector<double> atoms_host;  // per atom record, rism::atom_stride(natv) doubles
   rism_opts opts{};
   // device: the FP64 level, and the FP32 level of a mixed-precision context
   Level<double> d;
@@ -159,6 +160,7 @@ struct rism_ctx {
   int prec = 0;  // level the loop runs at: 0 FP64, 1 FP32
   double *partial = nullptr, *tpartial = nullptr, *sums = nullptr, *scratch = nullptr;
   double *atoms = nullptr, *knots = nullptr, *coef = nullptr, *chitab = nullptr;
+  size_t nchitab = 0;  // doubles in chitab
   size_t zinv_blocks = 0;
   std::vector<void *> allocs;
   // AN2 state (anderson.h:18-40)
@@ -339,6 +341,7 @@ int make_chi(rism_ctx *c, const rism_solvent *s) {
                       (long)(c->nz - 1) * (c->nz - 1);
     const size_t nshell = (size_t)((smax - 3) / 8 + 1);
     if ((rc = c->alloc(&c->chitab, nshell * nv2))) return rc;
+    c->nchitab = nshell * nv2;
     HIPCHECK(rism::run_chi_table(c->knots, c->coef, c->P, M_PI / c->box[0], nshell, c->chitab, c->st));
     HIPCHECK(hipStreamSynchronize(c->st));
     c->P.shell = 1;
@@ -616,7 +619,7 @@ int calc_step(rism_ctx *c, double *rms, bool lin = false) {
   }
   {
     KTime kt(c, RISM_K_XOZ);
-    HIPCHECK(rism::run_xoz(P, L.spec, L.fk, L.tb.twx, L.tb.prex, c->knots, c->coef, c->chitab,
+    HIPCHECK(rism::run_xoz(P, L.spec, L.fk, L.tb.twx, L.tb.prex, c->knots, c->coef, L.chitab,
                            c->st));
   }
   {
@@ -951,7 +954,7 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
 
   // solute table: cal_LJ.cu:26-40 mixing rules, -l scales sigma
   const double lam1 = (opts->ad_switch == 1) ? opts->lambda : 1.0;
-  const int stride = 4 + 2 * c->natv;
+  const int stride = rism::atom_stride(c->natv);
   c->atoms_host.assign((size_t)(c->natu > 0 ? c->natu : 1) * stride, 0.0);
   for (int a = 0; a < c->natu; ++a) {
     double *p = c->atoms_host.data() + (size_t)a * stride;
@@ -959,11 +962,17 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
     p[1] = solute->xyz[3 * a + 1];
     p[2] = solute->xyz[3 * a + 2];
     p[3] = solute->q[a];
-    for (int v = 0; v < c->natv; ++v) {
+    double rc2 = 0.0;
+    for (int v = 0; v < c->natv; ++v) {  // cal_LJ.cu:33-44: sigma_uv, 4 eps_uv
       const double su = (solute->sig[a] + c->sigv[v]) * 0.5 * lam1;
-      p[4 + v] = su * su;
-      p[4 + c->natv + v] = 4.0 * std::sqrt(solute->eps[a] * c->epsv[v]);
+      const double s2 = su * su, s6 = s2 * s2 * s2;
+      const double e4 = 4.0 * std::sqrt(solute->eps[a] * c->epsv[v]);
+      p[5 + v] = s2;
+      p[5 + c->natv + v] = e4 * s6 * s6;
+      p[5 + 2 * c->natv + v] = e4 * s6;
+      rc2 = std::max(rc2, 4.0e-6 * s2);
     }
+    p[4] = rc2;
   }
   if ((rc = upload(c, &c->atoms, c->atoms_host))) return fail(rc);
   if ((rc = make_tables(c, c->d.tb))) return fail(rc);
@@ -971,9 +980,17 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
 
   const size_t nn = c->N * c->natv;
   if ((rc = alloc_level(c, c->d, nullptr))) return fail(rc);
+  c->d.chitab = c->chitab;
   if (c->opts.precision == 1) {
     if ((rc = make_tables(c, c->f.tb))) return fail(rc);
     if ((rc = alloc_level(c, c->f, c->d.spec))) return fail(rc);
+    if (c->chitab) {
+      if ((rc = c->alloc(&c->f.chitab, c->nchitab))) return fail(rc);
+      if (rism::run_convert((const double *)c->chitab, c->f.chitab, c->nchitab, c->st) !=
+              hipSuccess ||
+          hipStreamSynchronize(c->st) != hipSuccess)
+        return fail(rism::set_error(RISM_ERR_HIP, "chi table conversion failed"));
+    }
   }
   if (c->opts.anderson_m > 0)
     c->D.assign((size_t)(c->opts.anderson_m + 1) * (c->opts.anderson_m + 1), 0.0);
@@ -1024,7 +1041,8 @@ int rism_setup_potentials(rism_ctx *c) {
   }
   if (do_coul && c->natu > 0) {
     KTime kt(c, RISM_K_FK);
-    HIPCHECK(rism::run_fk_half(c->atoms, c->natu, 4 + 2 * c->natv, c->P, ubeta, c->d.fk, c->st));
+    HIPCHECK(rism::run_fk_half(c->atoms, c->natu, rism::atom_stride(c->natv), c->P, ubeta,
+                               c->d.fk, c->st));
   } else {
     HIPCHECK(hipMemsetAsync(c->d.fk, 0, sizeof(cplx) * (c->N / 2), c->st));
   }
@@ -1396,7 +1414,8 @@ int rism_k_fft_r2c(int nx, int ny, int nz, const double box[3], const double *in
   P.mode = 1;
   HIPCHECK(rism::run_zfwd(P, t, t, t, spec, c->d.tb.twz, c->d.tb.zpre, c->d.tb.zpost, c->st));
   HIPCHECK(rism::run_y(P, spec, -1, c->d.tb.twy, c->d.tb.prey, c->st));
-  HIPCHECK(rism::run_xoz(P, spec, spec, c->d.tb.twx, c->d.tb.prex, nullptr, nullptr, nullptr, c->st));
+  HIPCHECK(rism::run_xoz(P, spec, spec, c->d.tb.twx, c->d.tb.prex, nullptr, nullptr,
+                         (const double *)nullptr, c->st));
   HIPCHECK(rism::run_apply_phase(spec, P, 0, c->st));
   HIPCHECK(hipMemcpyAsync(out, spec, sizeof(cplx) * (c->N / 2), hipMemcpyDeviceToHost, c->st));
   HIPCHECK(hipStreamSynchronize(c->st));
@@ -1422,7 +1441,8 @@ int rism_k_fft_c2r(int nx, int ny, int nz, const double box[3], const double *in
   rism::Params P = c->P;
   P.mode = 2;
   HIPCHECK(rism::run_apply_phase(spec, P, 1, c->st));
-  HIPCHECK(rism::run_xoz(P, spec, spec, c->d.tb.twx, c->d.tb.prex, nullptr, nullptr, nullptr, c->st));
+  HIPCHECK(rism::run_xoz(P, spec, spec, c->d.tb.twx, c->d.tb.prex, nullptr, nullptr,
+                         (const double *)nullptr, c->st));
   HIPCHECK(rism::run_y(P, spec, +1, c->d.tb.twy, c->d.tb.prey, c->st));
   HIPCHECK(rism::run_zinv(P, spec, h, h, h, h, h, (const double *)nullptr, partial, c->d.tb.twz,
                           c->d.tb.zpre, c->d.tb.zpost, 0, c->st));

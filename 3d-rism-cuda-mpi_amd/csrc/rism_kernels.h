@@ -46,6 +46,10 @@ struct HistUpdateT {
   int nh;
 };
 
+// doubles per solute atom record of the potential kernels: x, y, z, q,
+// rc2, sigma_uv^2[natv], 4 eps sigma^12 [natv], 4 eps sigma^6 [natv]
+inline __host__ __device__ int atom_stride(int natv) { return 5 + 3 * natv; }
+
 bool fft_length_supported(int L);
 template <class Re>
 size_t zinv_num_blocks(const Params &P);
@@ -61,10 +65,11 @@ hipError_t run_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<
 template <class Re>
 hipError_t run_y(const Params &P, cx<Re> *spec, int dir, const cx<Re> *twL, const cx<Re> *pre,
                  hipStream_t st);
+// chitab: the cubic-box shell table in the bulk type (P.shell = 1)
 template <class Re>
 hipError_t run_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const cx<Re> *twL,
                    const cx<Re> *prex, const double *knots, const double *coef,
-                   const double *chitab, hipStream_t st);
+                   const Re *chitab, hipStream_t st);
 hipError_t run_chi_table(const double *knots, const double *coef, const Params &P,
                          double kscale, size_t nshell, double *out, hipStream_t st);
 // AN2 partial sums (6 per workgroup) of the residual pass

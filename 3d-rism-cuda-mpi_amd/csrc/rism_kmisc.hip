@@ -143,8 +143,13 @@ __global__ void k_convert(const From *__restrict__ in, To *__restrict__ out, siz
 // ---------------------------------------------------------------------------
 // Each thread owns PX consecutive x points of one (y, z) line and a group of
 // G <= 8 solvent sites starting at v0; solute atoms stream through LDS in
-// tiles of ATILE records (x, y, z, q, sigma_uv^2[natv], 4 eps_uv[natv]).
-// Outputs u[v] = beta sum 4 eps (s^-12 - s^-6) + q_v de, and fr (first group).
+// tiles of ATILE records (atom_stride(natv) doubles: x, y, z, q, rc2,
+// sigma_uv^2[natv], A[natv] = 4 eps sigma^12, B[natv] = 4 eps sigma^6).
+// 4 eps (s^-12 - s^-6) with s^2 = r^2 / sigma^2 is A r^-12 - B r^-6, so the
+// powers of 1/r are formed once per pair and each site costs two FMAs.  The
+// reference's floor s^2 >= 4e-6 (cal_LJ.cu:11-12) can only bind when
+// r^2 < rc2 = 4e-6 max_v sigma_uv^2; those (rare) pairs take the clamped form.
+// Outputs u[v] = beta sum LJ + q_v de, and fr (first group).
 // 1/r comes from the v_rsq_f64 seed refined by two Newton steps, so the pair
 // loop has no FP64 division or square root.
 constexpr int PX = 4;
@@ -165,7 +170,7 @@ __global__ void __launch_bounds__(256)
                   double *__restrict__ fr) {
   extern __shared__ double sa[];
   const int nv = P.natv;
-  const int stride = 4 + 2 * nv;
+  const int stride = atom_stride(nv);
   const int nxq = (P.nx + PX - 1) / PX;
   const size_t lines = (size_t)P.ny * P.nz;
   const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,12 +201,12 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     for (int a = 0; a < na; ++a) {
       const double *at = sa + a * stride;
-      const double ax = at[0], ay = at[1], az = at[2], aq = at[3];
-      double sg2[G], e4[G];
+      const double ax = at[0], ay = at[1], az = at[2], aq = at[3], rc2 = at[4];
+      double A[G], B[G];
 #pragma unroll
       for (int w = 0; w < G; ++w) {
-        sg2[w] = at[4 + v0 + w];
-        e4[w] = at[4 + nv + v0 + w];
+        A[w] = at[5 + nv + v0 + w];
+        B[w] = at[5 + 2 * nv + v0 + w];
       }
       const double dy = ry - ay, dz = rz - az;
       const double dyz = dy * dy + dz * dz;
@@ -211,11 +216,20 @@ __global__ void __launch_bounds__(256)
         const double r2 = dx * dx + dyz;
         const double ir = rsq_nr(r2);  // NaN at r2 = 0: both uses below handle it
         const double ir2 = ir * ir;
+        if (r2 >= rc2) {
+          const double ir6 = ir2 * ir2 * ir2;
+          const double ir12 = ir6 * ir6;
 #pragma unroll
-        for (int w = 0; w < G; ++w) {
-          const double s2 = fmin(sg2[w] * ir2, icut2);  // fmin drops the NaN of r = 0
-          const double s6 = s2 * s2 * s2;
-          lj[p][w] += e4[w] * s6 * (s6 - 1.0);
+          for (int w = 0; w < G; ++w) lj[p][w] = fma(A[w], ir12, fma(-B[w], ir6, lj[p][w]));
+        } else {
+#pragma unroll
+          for (int w = 0; w < G; ++w) {
+            const double sg2 = at[5 + v0 + w];
+            const double e4 = B[w] / (sg2 * sg2 * sg2);
+            const double s2 = fmin(sg2 * ir2, icut2);  // fmin drops the NaN of r = 0
+            const double s6 = s2 * s2 * s2;
+            lj[p][w] += e4 * s6 * (s6 - 1.0);
+          }
         }
         if (do_coul) {
           const double ra = r2 * ir;
@@ -483,7 +497,7 @@ template hipError_t run_convert<float, double>(const float *, double *, size_t, 
 hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
                            double ubeta, int do_coul, double *u, double *fr, hipStream_t st) {
   const size_t nth = (size_t)P.ny * P.nz * ((P.nx + PX - 1) / PX);
-  const size_t lds = (size_t)ATILE * (4 + 2 * P.natv) * sizeof(double);
+  const size_t lds = (size_t)ATILE * atom_stride(P.natv) * sizeof(double);
   const dim3 grid(nblk(nth, 256)), block(256);
   for (int v0 = 0; v0 < P.natv; v0 += 8) {
     const int g = min(8, P.natv - v0);

@@ -39,9 +39,10 @@ namespace rism {
 // ChiSpline: the natural cubic spline of the .xvk table evaluated at |k|
 // (solvent_spline.cu:44-55 evaluates the same spline at the same |k| shell
 // values; splint.cc:4-20 picks the interval).
+template <class Tb>
 struct ChiTable {
-  const double *row;
-  __device__ __forceinline__ double operator()(int pw) const { return row[pw]; }
+  const Tb *row;
+  __device__ __forceinline__ Tb operator()(int pw) const { return row[pw]; }
 };
 struct ChiSpline {
   const double *cf;
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(512)
     k_xpass_oz(cx<Re> *__restrict__ spec, const cx<Re> *__restrict__ fkraw, Params P,
                const cx<Re> *__restrict__ twL, const cx<Re> *__restrict__ prex,
                const double *__restrict__ knots, const double *__restrict__ coef,
-               const double *__restrict__ chitab) {
+               const Re *__restrict__ chitab) {
   using C = cx<Re>;
   constexpr int T = FftShape<L>::T, CS = FftShape<L>::CS;
   C *lds = reinterpret_cast<C *>(xsm);
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(512)
       const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates
       const long ox = 2 * jx - L + 1;
       const long sidx = (ox * ox + oy * oy + oz * oz - 3) >> 3;  // |k|^2 = (pi/L)^2 (8 sidx + 3)
-      oz_row<NVMAX>(lds, CS, jx, L, fkr, P, ChiTable{chitab + sidx * nv2});
+      oz_row<NVMAX>(lds, CS, jx, L, fkr, P, ChiTable<Re>{chitab + sidx * nv2});
     } else {
       const double ky = P.kfac[1] * ((double)y - (double)P.ny / 2.0 + 0.5);
       const double kz = P.kfac[2] * ((double)(m + P.nz / 2) - (double)P.nz / 2.0 + 0.5);
@@ -212,7 +213,7 @@ __global__ void k_chi_table(const double *__restrict__ knots, const double *__re
 template <int L, class Re>
 static void launch_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const cx<Re> *twL,
                        const cx<Re> *prex, const double *knots, const double *coef,
-                       const double *chitab, hipStream_t st) {
+                       const Re *chitab, hipStream_t st) {
   dim3 grid(P.ny, P.nz / 2);
   const int threads = P.natv * FftShape<L>::T;
   const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cx<Re>);
@@ -253,7 +254,7 @@ int xoz_threads(int L, int natv) {
 template <class Re>
 hipError_t run_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const cx<Re> *twL,
                    const cx<Re> *prex, const double *knots, const double *coef,
-                   const double *chitab, hipStream_t st) {
+                   const Re *chitab, hipStream_t st) {
   switch (P.nx) {
 #define X(n)                                                           \
   case n:                                                              \
@@ -271,7 +272,7 @@ template hipError_t run_xoz<double>(const Params &, cplx *, const cplx *, const 
                                     hipStream_t);
 template hipError_t run_xoz<float>(const Params &, cplxf *, const cplxf *, const cplxf *,
                                    const cplxf *, const double *, const double *,
-                                   const double *, hipStream_t);
+                                   const float *, hipStream_t);
 hipError_t run_chi_table(const double *knots, const double *coef, const Params &P,
                           double kscale, size_t nshell, double *out, hipStream_t st) {
   hipLaunchKernelGGL(k_chi_table, dim3((unsigned)((nshell + 255) / 256)), dim3(256), 0, st, knots,
