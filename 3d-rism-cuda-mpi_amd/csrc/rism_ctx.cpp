@@ -14,6 +14,7 @@
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rism3d_c.h"
@@ -183,6 +184,13 @@ struct rism_ctx {
   std::vector<double> D;
   std::vector<double> lastth;
   std::vector<double> hsums;
+  // device-side step state: scal = {rms, x0, x1, s1, s2} of the last step,
+  // dflag = {done (1 converged, 2 diverged), step}, rms_hist[step]; hscal
+  // and hflag are their pinned host mirrors
+  double *scal = nullptr, *rms_hist = nullptr, *rho_dev = nullptr, *hscal = nullptr;
+  int *dflag = nullptr, *hflag = nullptr;
+  size_t rms_cap = 0;
+  bool async = false;
   bool have_step = false;
   rism_timing timing{};
   // per-kernel hipEvent timing (rism_set_kernel_timing)
@@ -210,6 +218,8 @@ struct rism_ctx {
     return RISM_OK;
   }
   ~rism_ctx() {
+    if (hscal) (void)hipHostFree(hscal);
+    if (hflag) (void)hipHostFree(hflag);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (void *q : allocs) (void)hipFree(q);
     if (st) (void)hipStreamDestroy(st);
@@ -512,7 +522,7 @@ int anderson_m_step(rism_ctx *c) {
   if (c->count > 0) {
     --c->count;
     KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_an2_picard(tnew, t, R, Pdst, c->opts.an2_m, nn, c->st));
+    HIPCHECK(rism::run_an2_picard(tnew, t, R, Pdst, c->opts.an2_m, nn, nullptr, c->st));
     c->lastth.clear();
   } else {
     std::vector<int> rb;
@@ -545,51 +555,24 @@ int anderson_m_step(rism_ctx *c) {
 }
 
 // One AN2::calculate (anderson.cu:43-78) at level Re: Picard warm-up, or
-// cal_theta + leg + the two-slot update, from the sums of the last step.
+// the two-slot update with the coefficients that the step's finish kernel
+// left in scal (cal_theta + leg on the device).
 template <class Re>
 int an2_step(rism_ctx *c) {
   Level<Re> &L = lv<Re>(c);
   const size_t nn = c->N * c->natv;
   const int b = c->binary;
   const int oldR = c->iR_slot[b];
+  const int *done = c->async ? c->dflag : nullptr;
   Re *t = L.T[c->iT_cur], *R = L.R[c->iR_cur], *tnew = L.T[1 - c->iT_cur];
   if (c->count > 0) {  // anderson.cu:54-60
     --c->count;
     KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_an2_picard(tnew, t, R, L.Pslot[b], c->opts.an2_m, nn, c->st));
-  } else {  // anderson.cu:61-72, cal_theta anderson.cu:81-102
-    double cc[2] = {0, 0}, a[4] = {0, 0, 0, 0};
-    for (int v = 0; v < c->natv; ++v) {
-      const double ir = 1.0 / c->rho[v];
-      const double *sv = &c->hsums[6 * v];
-      cc[0] += sv[1] * ir;
-      cc[1] += sv[2] * ir;
-      a[0] += sv[3] * ir;
-      a[1] += sv[4] * ir;
-      a[3] += sv[5] * ir;
-    }
-    a[2] = a[1];
-    // leg(): Gaussian elimination without pivoting (anderson.cu:105-130)
-    double x[2];
-    {
-      double p = a[0];
-      a[1] /= p;
-      double r = cc[0] /= p;
-      double q = a[2];
-      a[3] -= q * a[1];
-      cc[1] -= q * r;
-      x[1] = cc[1] / a[3];
-      x[0] = cc[0] - a[1] * x[1];
-    }
-    const double s1 = x[0] + c->opts.an2_mp * ((b == 0) ? 0 : 1);
-    const double s2 = x[1] + c->opts.an2_mp * ((b == 0) ? 1 : 0);
-    c->lastx[0] = x[0];
-    c->lastx[1] = x[1];
-    c->lasts[0] = s1;
-    c->lasts[1] = s2;
+    HIPCHECK(rism::run_an2_picard(tnew, t, R, L.Pslot[b], c->opts.an2_m, nn, done, c->st));
+  } else {  // anderson.cu:61-72
     KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_an2_update(tnew, t, R, L.Pslot[0], L.Pslot[1], L.Pslot[b], s1, s2,
-                                  c->opts.an2_m, nn, c->st));
+    HIPCHECK(rism::run_an2_update(tnew, t, R, L.Pslot[0], L.Pslot[1], L.Pslot[b], c->scal,
+                                  c->opts.an2_m, nn, done, c->st));
   }
   // slot[b] <- (t, R): P_b was rewritten by the kernel, R_b by pointer
   c->iR_slot[b] = c->iR_cur;
@@ -602,11 +585,14 @@ int an2_step(rism_ctx *c) {
 // One calculate() + cal_rms() at level Re (calculate.cu:4-65, cal_rms.cu:14-26)
 // lin: the difference-form step of a correction sweep (t = the correction
 // d, u = the closure exponent at t0, aux = the residual at t0).
+// In async mode (AN2 only) nothing is read back: the finish kernel records
+// rms_hist[step] and raises the done flag, which stops every later kernel.
 template <class Re>
-int calc_step(rism_ctx *c, double *rms, bool lin = false) {
+int calc_step(rism_ctx *c, double *rms, bool lin = false, int step = 0, double tol = -1.0) {
   Level<Re> &L = lv<Re>(c);
   rism::Params P = c->P;
   P.mode = lin ? 3 : 0;
+  P.done = c->async ? c->dflag : nullptr;
   Re *t = L.T[c->iT_cur];
   const Re *r0 = lin ? L.aux : nullptr;
   {
@@ -646,17 +632,27 @@ int calc_step(rism_ctx *c, double *rms, bool lin = false) {
   const size_t nb = rism::zinv_num_blocks<Re>(c->P) / c->natv;
   {
     KTime kt(c, RISM_K_REDUCE);
-    for (int v = 0; v < c->natv; ++v)
-      HIPCHECK(rism::run_reduce_cols(c->partial + (size_t)v * nb * W, nb, W, c->sums + W * v,
-                                     c->st));
+    HIPCHECK(rism::run_reduce_cols(c->partial, nb, W, c->natv, c->sums, P.done, c->st));
+    if (!andm)
+      HIPCHECK(rism::run_step_finish(c->sums, c->rho_dev, c->natv, c->N, tol, step,
+                                     c->count == 0, c->opts.an2_mp, c->binary, c->scal,
+                                     c->async ? c->rms_hist : nullptr,
+                                     c->async ? c->dflag : nullptr, c->st));
   }
-  HIPCHECK(hipMemcpyAsync(c->hsums.data(), c->sums, sizeof(double) * W * c->natv,
-                          hipMemcpyDeviceToHost, c->st));
-  HIPCHECK(hipStreamSynchronize(c->st));
-  if (c->ktime) ktime_resolve(c);
+  c->iT_last = c->iT_cur;
+  L.Rlast = Rout;
+  c->last_prec = c->prec;
+  c->fresh = true;
+  c->have_step = true;
+  if (c->async) return RISM_OK;
   double ss = 0.0;
-  for (int v = 0; v < c->natv; ++v) ss += c->hsums[W * v];
-  if (andm) {  // weighted dots of the new residual with itself and the history
+  if (andm) {
+    HIPCHECK(hipMemcpyAsync(c->hsums.data(), c->sums, sizeof(double) * W * c->natv,
+                            hipMemcpyDeviceToHost, c->st));
+    HIPCHECK(hipStreamSynchronize(c->st));
+    for (int v = 0; v < c->natv; ++v) ss += c->hsums[W * v];
+    ss = std::sqrt(ss / ((double)c->N * c->natv));  // cal_rms.cu:24
+    // weighted dots of the new residual with itself and the history
     const int m1 = c->opts.anderson_m + 1;
     auto wdot = [&](int k) {
       double a = 0.0;
@@ -668,13 +664,19 @@ int calc_step(rism_ctx *c, double *rms, bool lin = false) {
       const int bb = c->hist[p].rb;
       c->D[(size_t)c->cur_rb * m1 + bb] = c->D[(size_t)bb * m1 + c->cur_rb] = wdot(1 + (int)p);
     }
+  } else {
+    HIPCHECK(hipMemcpyAsync(c->hscal, c->scal, sizeof(double) * 5, hipMemcpyDeviceToHost, c->st));
+    HIPCHECK(hipStreamSynchronize(c->st));
+    ss = c->hscal[0];
+    if (c->count == 0) {
+      c->lastx[0] = c->hscal[1];
+      c->lastx[1] = c->hscal[2];
+      c->lasts[0] = c->hscal[3];
+      c->lasts[1] = c->hscal[4];
+    }
   }
-  c->iT_last = c->iT_cur;
-  L.Rlast = Rout;
-  c->last_prec = c->prec;
-  c->fresh = true;
-  c->have_step = true;
-  if (rms) *rms = std::sqrt(ss / ((double)c->N * c->natv));  // cal_rms.cu:24
+  if (c->ktime) ktime_resolve(c);
+  if (rms) *rms = ss;
   return RISM_OK;
 }
 
@@ -842,6 +844,139 @@ int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, b
   return RISM_OK;
 }
 
+// ---- solves without a host round trip per step (FP64, AN2) -------------------
+// The host enqueues steps in chunks; each step's finish kernel writes
+// rms_hist[step] and raises dflag at convergence or divergence, after which
+// every enqueued kernel of the context returns at entry.  The host reads the
+// flag once per chunk.  The arithmetic is that of rism_solve's step-by-step
+// loop, so the results are identical; only the iteration count is found
+// afterwards, from the flag.
+constexpr int kAsyncChunk = 16;  // steps enqueued between reads of the flag
+
+struct AsyncRun {
+  rism_ctx *c = nullptr;
+  int maxstep = 0, istep = 0;
+  double tol = 0.0;
+  bool finished = false;
+  int status = RISM_STATUS_MAXSTEP, steps = 0;
+  double rms = 0.0;
+  struct Log {
+    int iT;
+    double *R;
+  };
+  std::vector<Log> log;  // per step: t index and residual buffer
+};
+
+bool async_capable(const rism_ctx *c) {
+  return c->opts.anderson_m == 0 && c->opts.precision == 0 && !c->opts.verbose;
+}
+
+int async_begin(AsyncRun &a, rism_ctx *c, int maxstep, double tol) {
+  a = AsyncRun{};
+  a.c = c;
+  a.maxstep = maxstep;
+  a.tol = tol;
+  a.log.assign(maxstep + 1, AsyncRun::Log{0, nullptr});
+  if (c->rms_cap < (size_t)maxstep + 1) {
+    size_t cap = std::max<size_t>(1024, (size_t)maxstep + 1);
+    int rc = c->alloc(&c->rms_hist, cap);
+    if (rc) return rc;
+    c->rms_cap = cap;
+  }
+  HIPCHECK(hipMemsetAsync(c->dflag, 0, 2 * sizeof(int), c->st));
+  c->async = true;
+  return RISM_OK;
+}
+
+// enqueues up to `chunk` steps and the read-back of the flag
+int async_enqueue(AsyncRun &a, int chunk) {
+  rism_ctx *c = a.c;
+  int rc;
+  for (int k = 0; k < chunk && a.istep < a.maxstep; ++k) {
+    const int s = ++a.istep;
+    a.log[s] = AsyncRun::Log{c->iT_cur, c->d.R[c->iR_cur]};
+    if ((rc = calc_step<double>(c, nullptr, false, s, a.tol))) return rc;
+    c->fresh = false;
+    if ((rc = an2_step<double>(c))) return rc;
+  }
+  HIPCHECK(hipMemcpyAsync(c->hflag, c->dflag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  return RISM_OK;
+}
+
+// after the stream has drained: finishes the run if the flag is up or the
+// step limit is reached, restoring the state of the deciding step
+int async_check(AsyncRun &a) {
+  rism_ctx *c = a.c;
+  if (c->ktime) ktime_resolve(c);
+  const int flag = c->hflag[0];
+  if (!flag && a.istep < a.maxstep) return RISM_OK;
+  const int k = flag ? c->hflag[1] : a.maxstep;
+  a.finished = true;
+  a.steps = k;
+  a.status = flag == 1 ? RISM_STATUS_CONVERGED
+                       : (flag == 2 ? RISM_STATUS_DIVERGED : RISM_STATUS_MAXSTEP);
+  HIPCHECK(hipMemcpyAsync(c->hscal, c->rms_hist + k, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  a.rms = c->hscal[0];
+  c->async = false;
+  c->iT_last = a.log[k].iT;
+  c->d.Rlast = a.log[k].R;
+  c->last_prec = 0;
+  c->have_step = true;
+  c->fresh = false;
+  // converged or diverged at k: no update followed (iterate.cu:50-72); at
+  // the step limit the last update did, as in the step-by-step loop
+  if (flag) c->iT_cur = a.log[k].iT;
+  return RISM_OK;
+}
+
+int setup_enqueue(rism_ctx *c) {
+  const size_t nn = c->N * c->natv;
+  c->prec = 0;
+  reset_an2(c);
+  // history slots start at zero (the reference leaves them uninitialised)
+  if (c->opts.anderson_m == 0) {
+    for (int i = 0; i < 2; ++i) {
+      HIPCHECK(hipMemsetAsync(c->d.Pslot[i], 0, sizeof(double) * nn, c->st));
+      HIPCHECK(hipMemsetAsync(c->d.R[c->iR_slot[i]], 0, sizeof(double) * nn, c->st));
+      if (c->opts.precision == 1) {
+        HIPCHECK(hipMemsetAsync(c->f.Pslot[i], 0, sizeof(float) * nn, c->st));
+        HIPCHECK(hipMemsetAsync(c->f.R[c->iR_slot[i]], 0, sizeof(float) * nn, c->st));
+      }
+    }
+  }
+  if (c->opts.verbose) {
+    std::cout << "tabulating solute Lennard-Jones potential ..." << std::endl;
+    if (c->opts.ad_switch != 1) std::cout << "synthesizing solute Coulomb potential ..." << std::endl;
+  }
+  const double ikbt = 1.0 / (kAvogadro * kBoltzmann * c->temper);  // cal_LJ.cu:47
+  const double lam2 = (c->opts.ad_switch == 2) ? c->opts.lambda : 1.0;
+  const double ubeta = kHartree * kBohr / (kBoltzmann * c->temper) * lam2;  // cal_Coulomb.cu:33
+  const int do_coul = c->opts.ad_switch != 1;  // cal_Coulomb.cu:23
+  {
+    KTime kt(c, RISM_K_POT);
+    HIPCHECK(rism::run_potential_r(c->atoms, c->natu, c->P, ikbt, ubeta, do_coul, c->d.u,
+                                   c->d.fr, c->st));
+  }
+  if (do_coul && c->natu > 0) {
+    KTime kt(c, RISM_K_FK);
+    HIPCHECK(rism::run_fk_half(c->atoms, c->natu, rism::atom_stride(c->natv), c->P, ubeta,
+                               c->d.fk, c->st));
+  } else {
+    HIPCHECK(hipMemsetAsync(c->d.fk, 0, sizeof(cplx) * (c->N / 2), c->st));
+  }
+  if (c->opts.verbose) std::cout << "synthesizing initial estimate for Tuv ..." << std::endl;
+  HIPCHECK(rism::run_init_t(c->d.T[c->iT_cur], c->d.fr, c->P, c->N, c->st));
+  if (c->opts.precision == 1) {  // the FP32 level starts from the FP64 set-up
+    HIPCHECK(rism::run_convert(c->d.u, c->f.u, nn, c->st));
+    HIPCHECK(rism::run_convert(c->d.fr, c->f.fr, c->N, c->st));
+    HIPCHECK(rism::run_convert((const double *)c->d.fk, (float *)c->f.fk, c->N, c->st));
+    HIPCHECK(rism::run_convert(c->d.T[c->iT_cur], c->f.T[c->iT_cur], nn, c->st));
+    c->prec = c->last_prec = 1;
+  }
+  return RISM_OK;
+}
+
 // Device buffers of one level: u, fr, fk, t (ping-pong) and the AN2 or
 // Anderson(m) slots; the FP32 level's spectrum lives in the FP64 one's memory
 // (the two levels never run at the same time).
@@ -1001,6 +1136,12 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   if ((rc = c->alloc(&c->sums, kSumWidth * RISM_MAX_SITES))) return fail(rc);
   if ((rc = c->alloc(&c->scratch, c->N))) return fail(rc);
   c->hsums.assign(kSumWidth * c->natv, 0.0);
+  if ((rc = c->alloc(&c->scal, 8))) return fail(rc);
+  if ((rc = c->alloc(&c->dflag, 2))) return fail(rc);
+  if ((rc = upload(c, &c->rho_dev, c->rho))) return fail(rc);
+  if (hipHostMalloc((void **)&c->hscal, 8 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&c->hflag, 2 * sizeof(int)) != hipSuccess)
+    return fail(rism::set_error(RISM_ERR_NOMEM, "hipHostMalloc failed"));
   reset_an2(c);
   if (hipMemsetAsync(c->d.T[0], 0, sizeof(double) * nn, c->st) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
@@ -1012,49 +1153,8 @@ void rism_destroy(rism_ctx *ctx) { delete ctx; }
 
 int rism_setup_potentials(rism_ctx *c) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
-  const size_t nn = c->N * c->natv;
-  c->prec = 0;
-  reset_an2(c);
-  // history slots start at zero (the reference leaves them uninitialised)
-  if (c->opts.anderson_m == 0) {
-    for (int i = 0; i < 2; ++i) {
-      HIPCHECK(hipMemsetAsync(c->d.Pslot[i], 0, sizeof(double) * nn, c->st));
-      HIPCHECK(hipMemsetAsync(c->d.R[c->iR_slot[i]], 0, sizeof(double) * nn, c->st));
-      if (c->opts.precision == 1) {
-        HIPCHECK(hipMemsetAsync(c->f.Pslot[i], 0, sizeof(float) * nn, c->st));
-        HIPCHECK(hipMemsetAsync(c->f.R[c->iR_slot[i]], 0, sizeof(float) * nn, c->st));
-      }
-    }
-  }
-  if (c->opts.verbose) {
-    std::cout << "tabulating solute Lennard-Jones potential ..." << std::endl;
-    if (c->opts.ad_switch != 1) std::cout << "synthesizing solute Coulomb potential ..." << std::endl;
-  }
-  const double ikbt = 1.0 / (kAvogadro * kBoltzmann * c->temper);  // cal_LJ.cu:47
-  const double lam2 = (c->opts.ad_switch == 2) ? c->opts.lambda : 1.0;
-  const double ubeta = kHartree * kBohr / (kBoltzmann * c->temper) * lam2;  // cal_Coulomb.cu:33
-  const int do_coul = c->opts.ad_switch != 1;  // cal_Coulomb.cu:23
-  {
-    KTime kt(c, RISM_K_POT);
-    HIPCHECK(rism::run_potential_r(c->atoms, c->natu, c->P, ikbt, ubeta, do_coul, c->d.u,
-                                   c->d.fr, c->st));
-  }
-  if (do_coul && c->natu > 0) {
-    KTime kt(c, RISM_K_FK);
-    HIPCHECK(rism::run_fk_half(c->atoms, c->natu, rism::atom_stride(c->natv), c->P, ubeta,
-                               c->d.fk, c->st));
-  } else {
-    HIPCHECK(hipMemsetAsync(c->d.fk, 0, sizeof(cplx) * (c->N / 2), c->st));
-  }
-  if (c->opts.verbose) std::cout << "synthesizing initial estimate for Tuv ..." << std::endl;
-  HIPCHECK(rism::run_init_t(c->d.T[c->iT_cur], c->d.fr, c->P, c->N, c->st));
-  if (c->opts.precision == 1) {  // the FP32 level starts from the FP64 set-up
-    HIPCHECK(rism::run_convert(c->d.u, c->f.u, nn, c->st));
-    HIPCHECK(rism::run_convert(c->d.fr, c->f.fr, c->N, c->st));
-    HIPCHECK(rism::run_convert((const double *)c->d.fk, (float *)c->f.fk, c->N, c->st));
-    HIPCHECK(rism::run_convert(c->d.T[c->iT_cur], c->f.T[c->iT_cur], nn, c->st));
-    c->prec = c->last_prec = 1;
-  }
+  int rc = setup_enqueue(c);
+  if (rc) return rc;
   HIPCHECK(hipStreamSynchronize(c->st));
   return RISM_OK;
 }
@@ -1087,6 +1187,27 @@ int rism_get_precision(rism_ctx *c) { return c ? c->prec : -1; }
 
 int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, double *rms_out) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
+  if (async_capable(c) && !c->ktime && c->prec == 0 && maxstep > 0) {
+    // same steps without a host round trip per step
+    AsyncRun a;
+    int rc = async_begin(a, c, maxstep, tol);
+    while (!rc && !a.finished) {
+      rc = async_enqueue(a, kAsyncChunk);
+      if (!rc && hipStreamSynchronize(c->st) != hipSuccess)
+        rc = rism::set_error(RISM_ERR_HIP, "hipStreamSynchronize failed");
+      if (!rc) rc = async_check(a);
+    }
+    c->async = false;
+    if (rc) return rc;
+    if (status) *status = a.status;
+    if (steps) *steps = a.steps;
+    if (rms_out) *rms_out = a.rms;
+    c->timing.steps = a.steps;
+    c->timing.status = a.status;
+    c->timing.rms = a.rms;
+    c->timing.steps_fp32 = 0;
+    return RISM_OK;
+  }
   if (c->opts.verbose) std::cout << "relaxing 3D UV RISM:" << std::endl;
   bool conver = false, diverge = false;
   int istep = 0, last = 0, low_steps = 0;
@@ -1148,8 +1269,7 @@ int rism_thermo(rism_ctx *c, rism_thermo_out *out) {
       HIPCHECK(rism::run_thermo_partial(c->d.T[c->iT_cur], c->d.T[c->iT_last], c->d.u, c->d.Rlast,
                                       c->P, c->N, c->tpartial, c->st));
   }
-  for (int v = 0; v < c->natv; ++v)
-    HIPCHECK(rism::run_reduce_cols(c->tpartial + (size_t)v * tb * 4, tb, 4, c->sums + 4 * v, c->st));
+  HIPCHECK(rism::run_reduce_cols(c->tpartial, tb, 4, c->natv, c->sums, nullptr, c->st));
   std::vector<double> s(4 * c->natv);
   HIPCHECK(hipMemcpyAsync(s.data(), c->sums, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHECK(hipStreamSynchronize(c->st));
@@ -1270,6 +1390,76 @@ int rism_set_field(rism_ctx *c, int which, int site, const double *host) {
 int rism_get_timing(rism_ctx *c, rism_timing *out) {
   if (!c || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
   *out = c->timing;
+  return RISM_OK;
+}
+
+int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads, int *status,
+                     int *steps, double *rms) {
+  if (!ctx || n < 0) return rism::set_error(RISM_ERR_ARG, "null argument");
+  for (int i = 0; i < n; ++i) {
+    if (!ctx[i]) return rism::set_error(RISM_ERR_ARG, "null context in batch");
+    if (!async_capable(ctx[i]) || ctx[i]->prec != 0)
+      return rism::set_error(RISM_ERR_ARG,
+                             "batched solves run FP64 AN2 contexts without verbose output");
+  }
+  if (maxstep <= 0) return rism::set_error(RISM_ERR_ARG, "maxstep must be positive");
+  if (n == 0) return RISM_OK;
+  std::vector<AsyncRun> runs(n);
+  const int nt = std::max(1, std::min(threads > 0 ? threads : 1, n));
+  std::vector<int> rcs(nt, RISM_OK);
+  // worker w drives contexts w, w + nt, ...: enqueue a chunk on each of its
+  // streams, then drain them in turn, until every run has finished
+  auto work = [&](int w) {
+    int rc = RISM_OK;
+    for (int i = w; i < n && !rc; i += nt) rc = async_begin(runs[i], ctx[i], maxstep, tol);
+    bool left = true;
+    while (!rc && left) {
+      left = false;
+      for (int i = w; i < n && !rc; i += nt)
+        if (!runs[i].finished) rc = async_enqueue(runs[i], kAsyncChunk);
+      for (int i = w; i < n && !rc; i += nt) {
+        if (runs[i].finished) continue;
+        if (hipStreamSynchronize(ctx[i]->st) != hipSuccess)
+          rc = rism::set_error(RISM_ERR_HIP, "hipStreamSynchronize failed");
+        if (!rc) rc = async_check(runs[i]);
+        left = left || !runs[i].finished;
+      }
+    }
+    for (int i = w; i < n; i += nt) ctx[i]->async = false;
+    rcs[w] = rc;
+  };
+  std::vector<std::thread> pool;
+  for (int w = 1; w < nt; ++w) pool.emplace_back(work, w);
+  work(0);
+  for (auto &th : pool) th.join();
+  for (int w = 0; w < nt; ++w)
+    if (rcs[w]) return rcs[w];
+  for (int i = 0; i < n; ++i) {
+    if (status) status[i] = runs[i].status;
+    if (steps) steps[i] = runs[i].steps;
+    if (rms) rms[i] = runs[i].rms;
+    ctx[i]->timing.steps = runs[i].steps;
+    ctx[i]->timing.status = runs[i].status;
+    ctx[i]->timing.rms = runs[i].rms;
+  }
+  return RISM_OK;
+}
+
+int rism_run_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads,
+                   rism_thermo_out *out, int *status, int *steps) {
+  if (!ctx || n < 0) return rism::set_error(RISM_ERR_ARG, "null argument");
+  for (int i = 0; i < n; ++i) {
+    if (!ctx[i]) return rism::set_error(RISM_ERR_ARG, "null context in batch");
+    int rc = setup_enqueue(ctx[i]);
+    if (rc) return rc;
+  }
+  int rc = rism_solve_batch(ctx, n, maxstep, tol, threads, status, steps, nullptr);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) {
+    rism_thermo_out th;
+    if ((rc = rism_thermo(ctx[i], &th))) return rc;
+    if (out) out[i] = th;
+  }
   return RISM_OK;
 }
 

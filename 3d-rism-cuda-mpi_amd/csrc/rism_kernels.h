@@ -30,6 +30,9 @@ struct Params {
   // chi spline: knots x_i = x0 + i/idk approximately (exact values in `knots`)
   int np;
   double x0, idk;
+  // device flag of a solve driven without host synchronisation: once it is
+  // non-zero (converged or diverged) every step kernel returns at entry
+  const int *done;
 };
 
 // History of the Anderson(m) accelerator passed to kernels by value.
@@ -89,14 +92,25 @@ hipError_t run_zinv_hist(const Params &P, const cx<Re> *spec, const Re *t, const
 template <class Re>
 hipError_t run_andm_update(Re *tnew, const Re *t, const Re *R, const HistUpdateT<Re> &H, Re *Pb,
                            double beta, size_t n, hipStream_t st);
-hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out,
-                           hipStream_t st);
+// nsets consecutive blocks of nrow x width partial sums -> nsets x width
+hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, double *out,
+                           const int *done, hipStream_t st);
+// End of one AN2 step on the device (cal_rms.cu:20-24 and, when the next
+// update mixes, cal_theta + leg of anderson.cu:81-130): from the per-site
+// sums (6 per site) it writes scal = {rms, x0, x1, s1, s2}, rms_hist[step]
+// (if non-null) and, when rms <= tol or rms is not finite, done = {1 or 2,
+// step}.  binary: the slot the next update retires.
+hipError_t run_step_finish(const double *sums, const double *rho, int natv, size_t N,
+                           double tol, int step, int mixing, double mp, int binary,
+                           double *scal, double *rms_hist, int *done, hipStream_t st);
 template <class Re>
 hipError_t run_an2_picard(Re *tnew, const Re *t, const Re *R, Re *Pb, double m, size_t n,
-                          hipStream_t st);
+                          const int *done, hipStream_t st);
+// s1, s2 from scal[3], scal[4] (run_step_finish)
 template <class Re>
 hipError_t run_an2_update(Re *tnew, const Re *t, const Re *R, const Re *P0, const Re *P1, Re *Pb,
-                          double s1, double s2, double m, size_t n, hipStream_t st);
+                          const double *scal, double m, size_t n, const int *done,
+                          hipStream_t st);
 template <class From, class To>
 hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st);
 // closure exponent a = -u + t at the base point t0 of a correction sweep,

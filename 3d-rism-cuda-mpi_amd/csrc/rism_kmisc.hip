@@ -7,13 +7,17 @@ namespace rism {
 // ---------------------------------------------------------------------------
 // FP64 reductions
 // ---------------------------------------------------------------------------
-// Sums nblk rows of `width` doubles into out[width] (one workgroup per column
-// group; pairwise tree within the workgroup, sequential across strides).
+// Sums nrow rows of `width` doubles into out[width], for each of
+// gridDim.y consecutive blocks of rows (one workgroup per column and block;
+// Kahan-compensated per thread, pairwise tree within the workgroup).
 __global__ void __launch_bounds__(256)
     k_reduce_cols(const double *__restrict__ in, size_t nrow, int width,
-                  double *__restrict__ out) {
+                  double *__restrict__ out, const int *done) {
   __shared__ double sh[256];
+  if (done && *done) return;
   const int col = blockIdx.x;
+  in += (size_t)blockIdx.y * nrow * width;
+  out += (size_t)blockIdx.y * width;
   double a = 0.0, comp = 0.0;  // Kahan-compensated per-thread accumulation
   for (size_t r = threadIdx.x; r < nrow; r += blockDim.x) {
     const double yv = in[r * width + col] - comp;
@@ -57,7 +61,8 @@ struct alignas(16) V16 {
 template <class Re>
 __global__ void k_an2_picard(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
                              const V16<Re> *__restrict__ R, V16<Re> *__restrict__ Pb, Re mcoef,
-                             size_t nv) {
+                             size_t nv, const int *done) {
+  if (done && *done) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const V16<Re> ti = t[i], ri = R[i];
@@ -76,9 +81,12 @@ __global__ void k_an2_picard(V16<Re> *__restrict__ tnew, const V16<Re> *__restri
 template <class Re>
 __global__ void k_an2_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
                              const V16<Re> *__restrict__ R, const V16<Re> *P0,
-                             const V16<Re> *P1, V16<Re> *Pb, Re s1, Re s2, Re mcoef, size_t nv) {
+                             const V16<Re> *P1, V16<Re> *Pb, const double *__restrict__ scal,
+                             Re mcoef, size_t nv, const int *done) {
+  if (done && *done) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
+  const Re s1 = (Re)scal[3], s2 = (Re)scal[4];
   const V16<Re> ti = t[i], ri = R[i], p0 = P0[i], p1 = P1[i];
   V16<Re> a, q;
 #pragma unroll
@@ -88,6 +96,58 @@ __global__ void k_an2_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restri
   }
   tnew[i] = a;
   Pb[i] = q;
+}
+
+// End of a step: rms (cal_rms.cu:20-24), convergence / divergence flag
+// (iterate.cu:50-57) and the AN2 mixing coefficients for the next update
+// (cal_theta anderson.cu:81-102 with the rho^-1 weights, leg()'s Gaussian
+// elimination without pivoting anderson.cu:105-130, the mp shift
+// anderson.cu:66-67).  One thread; the sums are 6 doubles per site.
+__global__ void k_step_finish(const double *__restrict__ sums, const double *__restrict__ rho,
+                              int natv, size_t N, double tol, int step, int mixing, double mp,
+                              int binary, double *__restrict__ scal, double *__restrict__ rms_hist,
+                              int *__restrict__ done) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (done && done[0]) return;
+  double ss = 0.0, cc0 = 0.0, cc1 = 0.0, a0 = 0.0, a1 = 0.0, a3 = 0.0;
+  for (int v = 0; v < natv; ++v) {
+    const double *s = sums + 6 * v;
+    const double ir = 1.0 / rho[v];
+    ss += s[0];
+    cc0 += s[1] * ir;
+    cc1 += s[2] * ir;
+    a0 += s[3] * ir;
+    a1 += s[4] * ir;
+    a3 += s[5] * ir;
+  }
+  const double rms = sqrt(ss / ((double)N * natv));
+  scal[0] = rms;
+  if (rms_hist) rms_hist[step] = rms;
+  if (done) {
+    if (!isfinite(rms)) {
+      done[0] = 2;
+      done[1] = step;
+    } else if (rms <= tol) {
+      done[0] = 1;
+      done[1] = step;
+    }
+  }
+  if (mixing) {
+    const double a2 = a1;
+    double b1 = a1, b3 = a3, c0 = cc0, c1 = cc1;
+    const double p = a0;
+    b1 /= p;
+    const double r = c0 /= p;
+    const double q = a2;
+    b3 -= q * b1;
+    c1 -= q * r;
+    const double x1 = c1 / b3;
+    const double x0 = c0 - b1 * x1;
+    scal[1] = x0;
+    scal[2] = x1;
+    scal[3] = x0 + mp * ((binary == 0) ? 0 : 1);
+    scal[4] = x1 + mp * ((binary == 0) ? 1 : 0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -423,9 +483,18 @@ __global__ void k_apply_phase(cplx *__restrict__ s, Params P, int inverse) {
   s[i] = inverse ? mk(-sg * a.y, sg * a.x) : mk(sg * a.y, -sg * a.x);
 }
 
-hipError_t run_reduce_cols(const double *in, size_t nrow, int width, double *out,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_cols, dim3(width), dim3(256), 0, st, in, nrow, width, out);
+hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, double *out,
+                           const int *done, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_cols, dim3(width, nsets), dim3(256), 0, st, in, nrow, width, out,
+                     done);
+  return hipGetLastError();
+}
+
+hipError_t run_step_finish(const double *sums, const double *rho, int natv, size_t N,
+                           double tol, int step, int mixing, double mp, int binary,
+                           double *scal, double *rms_hist, int *done, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(64), 0, st, sums, rho, natv, N, tol, step,
+                     mixing, mp, binary, scal, rms_hist, done);
   return hipGetLastError();
 }
 
@@ -436,22 +505,23 @@ static size_t nvec(size_t n) {
 }
 template <class Re>
 hipError_t run_an2_picard(Re *tnew, const Re *t, const Re *R, Re *Pb, double m, size_t n,
-                          hipStream_t st) {
+                          const int *done, hipStream_t st) {
   if (n % V16<Re>::K) return hipErrorInvalidValue;
   const size_t nv = nvec<Re>(n);
   hipLaunchKernelGGL(k_an2_picard<Re>, dim3(nblk(nv, 256)), dim3(256), 0, st, (V16<Re> *)tnew,
-                     (const V16<Re> *)t, (const V16<Re> *)R, (V16<Re> *)Pb, (Re)m, nv);
+                     (const V16<Re> *)t, (const V16<Re> *)R, (V16<Re> *)Pb, (Re)m, nv, done);
   return hipGetLastError();
 }
 
 template <class Re>
 hipError_t run_an2_update(Re *tnew, const Re *t, const Re *R, const Re *P0, const Re *P1, Re *Pb,
-                          double s1, double s2, double m, size_t n, hipStream_t st) {
+                          const double *scal, double m, size_t n, const int *done,
+                          hipStream_t st) {
   if (n % V16<Re>::K) return hipErrorInvalidValue;
   const size_t nv = nvec<Re>(n);
   hipLaunchKernelGGL(k_an2_update<Re>, dim3(nblk(nv, 256)), dim3(256), 0, st, (V16<Re> *)tnew,
                      (const V16<Re> *)t, (const V16<Re> *)R, (const V16<Re> *)P0,
-                     (const V16<Re> *)P1, (V16<Re> *)Pb, (Re)s1, (Re)s2, (Re)m, nv);
+                     (const V16<Re> *)P1, (V16<Re> *)Pb, scal, (Re)m, nv, done);
   return hipGetLastError();
 }
 
@@ -473,9 +543,10 @@ hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st) {
 
 #define INST(Re)                                                                              \
   template hipError_t run_an2_picard<Re>(Re *, const Re *, const Re *, Re *, double, size_t,  \
-                                         hipStream_t);                                        \
+                                         const int *, hipStream_t);                           \
   template hipError_t run_an2_update<Re>(Re *, const Re *, const Re *, const Re *, const Re *, \
-                                         Re *, double, double, double, size_t, hipStream_t);  \
+                                         Re *, const double *, double, size_t, const int *,   \
+                                         hipStream_t);                                        \
   template hipError_t run_andm_update<Re>(Re *, const Re *, const Re *,                       \
                                           const HistUpdateT<Re> &, Re *, double, size_t,      \
                                           hipStream_t);                                       \

@@ -141,6 +141,7 @@ __global__ void __launch_bounds__(512)
   using C = cx<Re>;
   constexpr int T = FftShape<L>::T, CS = FftShape<L>::CS;
   C *lds = reinterpret_cast<C *>(xsm);
+  if (P.done && *P.done) return;
   const int tid = threadIdx.x;
   const int nv = P.natv;
   const int v = tid / T, j = tid % T;

@@ -11,10 +11,11 @@ namespace rism {
 template <int L, int B, int S, class Re>
 __global__ void __launch_bounds__(B *FftShape<L>::T)
     k_ypass(cx<Re> *__restrict__ spec, int nx, const cx<Re> *__restrict__ twL,
-            const cx<Re> *__restrict__ pre) {
+            const cx<Re> *__restrict__ pre, const int *done) {
   using C = cx<Re>;
   constexpr int CS = FftShape<L>::CS;
   __shared__ C lds[B * CS];
+  if (done && *done) return;
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   const int x = blockIdx.x * B + c;
@@ -37,7 +38,7 @@ static void launch_y(const Params &P, cx<Re> *spec, int nplanes, const cx<Re> *t
   constexpr int B = tile_cols<L, Re>();
   dim3 grid((P.nx + B - 1) / B, nplanes);
   hipLaunchKernelGGL((k_ypass<L, B, S, Re>), grid, dim3(B * FftShape<L>::T), 0, st, spec, P.nx,
-                     twL, pre);
+                     twL, pre, P.done);
 }
 template <class Re>
 hipError_t run_y(const Params &P, cx<Re> *spec, int dir, const cx<Re> *twL, const cx<Re> *pre,

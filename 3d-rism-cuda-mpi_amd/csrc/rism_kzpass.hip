@@ -23,6 +23,7 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   using C = cx<Re>;
   constexpr int T = FftShape<M>::T, CS = FftShape<M>::CS;
   __shared__ C lds[B * CS];
+  if (P.done && *P.done) return;
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   // logical workgroup order: site fastest, then x tile, then y (sites of
@@ -103,6 +104,7 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
   constexpr int NC = (int)((NRED * W * sizeof(double) + sizeof(C) - 1) / sizeof(C));
   constexpr int LDSN = (B * CS > NC) ? B * CS : NC;
   __shared__ C lds[LDSN];
+  if (P.done && *P.done) return;
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   const int ntx = (P.nx + B - 1) / B;

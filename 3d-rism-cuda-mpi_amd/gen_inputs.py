@@ -128,3 +128,30 @@ def cfg2_solute():
 def cfg1_solute():
     """24-atom random-walk solute, net charge 0."""
     return random_walk_solute(24, seed=1001, total_charge=0.0)
+
+
+# per-axis grid sizes of the batched small-solute workload (all 5-smooth,
+# multiples of 4) and its spacing and padding
+BATCH_SIZES = (48, 60, 64, 72, 80, 96)
+BATCH_SPACING = 0.5
+BATCH_PAD = 12.0
+
+
+def batch_solute(i, seed0=1004000):
+    """Solute i of the batched workload: 8-60 atoms (uniform), random walk,
+    net charge -1, 0 or +1 (uniform), from seed seed0 + i."""
+    rng = np.random.default_rng(seed0 + i)
+    n = int(rng.integers(8, 61))
+    q = float(rng.integers(-1, 2))
+    return random_walk_solute(n, seed=seed0 + i, total_charge=q)
+
+
+def batch_grid(xyz, sizes=BATCH_SIZES, spacing=BATCH_SPACING, pad=BATCH_PAD):
+    """Per axis the smallest size whose box (size * spacing) holds the solute
+    extent plus `pad` on both sides; the largest size when none does."""
+    ext = xyz.max(0) - xyz.min(0)
+    grid = []
+    for e in ext:
+        fit = [n for n in sizes if n * spacing >= e + 2 * pad]
+        grid.append(fit[0] if fit else sizes[-1])
+    return tuple(grid), tuple(n * spacing for n in grid)

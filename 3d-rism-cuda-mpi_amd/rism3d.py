@@ -41,6 +41,7 @@ EXPORTS = [
     "rism_write_guv", "rism_write_tuv", "rism_read_tuv", "rism_k_fft_r2c",
     "rism_k_fft_c2r", "rism_k_get_fk", "rism_k_an2_state", "rism_set_kernel_timing",
     "rism_get_kernel_stats", "rism_set_precision", "rism_get_precision",
+    "rism_solve_batch", "rism_run_batch",
 ]
 KERNELS = ["zfwd_closure", "ypass_fwd", "xpass_oz", "ypass_inv", "zinv_resid", "reduce",
            "an2_update", "potential_r", "fk_half", "thermo_partial"]
@@ -146,6 +147,9 @@ def lib():
     L.rism_calculate.argtypes = [vp, _dp]
     L.rism_anderson.argtypes = [vp]
     L.rism_set_precision.argtypes = [vp, ci]
+    L.rism_solve_batch.argtypes = [ctypes.POINTER(vp), ci, ci, cd, ci, pi, pi, _dp]
+    L.rism_run_batch.argtypes = [ctypes.POINTER(vp), ci, ci, cd, ci, ctypes.POINTER(Thermo),
+                                 pi, pi]
     L.rism_get_precision.argtypes = [vp]
     L.rism_solve.argtypes = [vp, ci, cd, pi, pi, _dp]
     L.rism_thermo.argtypes = [vp, ctypes.POINTER(Thermo)]
@@ -268,6 +272,33 @@ def fft_c2r(F, box):
 # ---------------------------------------------------------------------------
 # the solver
 # ---------------------------------------------------------------------------
+
+def solve_batch(solvers, maxstep=10000, tol=1e-6, threads=8):
+    """rism_solve on several RISM3D objects at once (after initialize());
+    returns [(status, steps, rms)]."""
+    n = len(solvers)
+    arr = (ctypes.c_void_p * n)(*[g._ctx for g in solvers])
+    st, sp = (ctypes.c_int * n)(), (ctypes.c_int * n)()
+    rms = np.zeros(n)
+    _check(lib().rism_solve_batch(arr, n, int(maxstep), float(tol), int(threads), st, sp,
+                                  _ptr(rms)))
+    return [(st[i], sp[i], rms[i]) for i in range(n)]
+
+
+def run_batch(solvers, maxstep=10000, tol=1e-6, threads=8):
+    """Potentials + batched solve + thermodynamics; returns [(thermo dict,
+    status, steps)]."""
+    n = len(solvers)
+    arr = (ctypes.c_void_p * n)(*[g._ctx for g in solvers])
+    th = (Thermo * n)()
+    st, sp = (ctypes.c_int * n)(), (ctypes.c_int * n)()
+    _check(lib().rism_run_batch(arr, n, int(maxstep), float(tol), int(threads), th, st, sp))
+    out = []
+    for i, g in enumerate(solvers):
+        g.last_thermo = th[i]
+        out.append((th[i].as_dict(g.closure), st[i], sp[i]))
+    return out
+
 
 class RISM3D:
     """One 3D-RISM solve on the GPU (RISM3D of rism3d.h:21-104).

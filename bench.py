@@ -158,12 +158,117 @@ def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv, budget_s=30.0):
     }
 
 
+def bench_batch(args):
+    """Batched small solutes: 256 random-walk solutes of 8-60 atoms (seeds
+    1004000 + i, net charge -1/0/+1) in water (bundled h2o_vv.xvk chi), each in
+    its own box of extent + 12 A padding per side at 0.5 A spacing, per-axis
+    grid sizes from {48, 60, 64, 72, 80, 96}; KH, AN2 2 0.6 0.2, tol 1e-6,
+    FP64.  Times the batch (rism_run_batch, one stream per solute, host
+    threads enqueueing) against the same solves run one after another."""
+    import torch
+    import gen_inputs as G
+    import __graft_entry__ as ge
+    rism = ge.load_rism3d()
+    solv = rism.read_xvk(os.path.join(GOLDEN, "h2o_vv.xvk"))
+    nsol = args.batch_size
+    cases = []
+    for i in range(nsol):
+        sol = G.batch_solute(i)
+        grid, box = G.batch_grid(sol["xyz"])
+        xyz = sol["xyz"]
+        mn, mx = xyz.min(0), xyz.max(0)
+        shift = np.array([np.floor(-(mx[k] - mn[k]) / 2 - mn[k] + 0.5) for k in range(3)])
+        cases.append((grid, box, {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"],
+                                  "uxyz": xyz + shift}))
+    log(f"[bench] batch: {nsol} solutes, {len(set(c[0] for c in cases))} grid shapes")
+
+    def make():
+        return [rism.RISM3D(g, b, solv, su, rism.KH, 2, 0.6, 0.2) for g, b, su in cases]
+
+    # warm-up (module load, first launches)
+    w = make()[:4]
+    rism.run_batch(w, threads=args.batch_threads)
+    for g in w:
+        g.close()
+    # batched
+    gs = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = rism.run_batch(gs, threads=args.batch_threads)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    steps = np.array([o[2] for o in out])
+    status = [o[1] for o in out]
+    sfe_batch = [o[0]["SFE_SC"] for o in out]
+    for g in gs:
+        g.close()
+    # the same solves one after another
+    gs = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sfe_seq = []
+    for g in gs:
+        th, tm = g.run()
+        sfe_seq.append(th["SFE_SC"])
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    for g in gs:
+        g.close()
+    same = all(a == b for a, b in zip(sfe_batch, sfe_seq))
+    # the same solves one after another with a host round trip per step (the
+    # reference's loop structure: rms read back before every update)
+    gs = make()
+    for g in gs:
+        g.kernel_timing(True)  # per-kernel events: forces the step-by-step loop
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for g in gs:
+        g.run()
+    torch.cuda.synchronize()
+    t_step = time.perf_counter() - t0
+    for g in gs:
+        g.close()
+    res = {
+        "metric": METRIC,
+        "value": nsol / t_batch,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1000.0 * t_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("batch: %d random-walk solutes (8-60 atoms, seeds 1004000+i, net "
+                         "charge -1/0/+1) in water (bundled h2o_vv.xvk chi), boxes = extent + "
+                         "12 A/side at 0.5 A, grids from {48..96}^3, KH, AN2 2 0.6 0.2, "
+                         "tol 1e-6, FP64" % nsol),
+            "solves": nsol, "grid_shapes": len(set(c[0] for c in cases)),
+            "batch_s": t_batch, "sequential_s": t_seq,
+            "sequential_solves_per_s": nsol / t_seq,
+            "speedup_vs_sequential": t_seq / t_batch,
+            "sequential_stepwise_s": t_step,
+            "speedup_vs_sequential_stepwise": t_step / t_batch,
+            "iterations_p50": float(np.percentile(steps, 50)),
+            "iterations_p99": float(np.percentile(steps, 99)),
+            "converged": int(sum(1 for s in status if s == rism.STATUS_CONVERGED)),
+            "batch_equals_sequential": bool(same),
+            "host_threads": args.batch_threads,
+        },
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--workload", default="cfg2",
+                    help="cfg2 (512^3, default), cfg1, mobley, or batch (256 small solutes)")
+    ap.add_argument("--batch-size", type=int, default=256)
+    ap.add_argument("--batch-threads", type=int, default=8)
     ap.add_argument("--anderson-m", type=int, default=None,
                     help="history length of Anderson(m) mixing (0: the reference's AN2)")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
@@ -196,6 +301,9 @@ def main():
         print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 5}), flush=True)
         return
 
+    if args.workload == "batch":
+        bench_batch(args)
+        return
     rism, grid, box, solv, sol, xvk, desc = workload(args.workload)
     solu = {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"], "uxyz": sol["xyz"]}
     log(f"[bench] workload {args.workload}: grid {grid}, natu {len(sol['sig'])}, "

@@ -169,6 +169,21 @@ int rism_calculate(rism_ctx *ctx, double *rms);
 /* Replaces AN2::calculate (anderson.cu:43-78): Picard warm-up or the
  * two-slot modified Anderson update of t from the last residual. */
 int rism_anderson(rism_ctx *ctx);
+/* ---- batches of independent solves (no counterpart in the reference,
+ * which runs one solute per process) ------------------------------------ */
+/* Runs rism_solve on n contexts at once: each context's steps are enqueued
+ * on its own stream in chunks, without a host round trip per step (a device
+ * flag stops a context's kernels at convergence), from `threads` host
+ * threads.  The results are those of n separate rism_solve calls.  The
+ * contexts must be FP64 (precision 0) with AN2 (anderson_m 0) and verbose
+ * 0.  status/steps/rms: n entries each (may be NULL). */
+int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads,
+                     int *status, int *steps, double *rms);
+/* Potentials, batched solve and thermodynamics for n contexts (rism_run for
+ * a batch); out, status, steps: n entries each (may be NULL). */
+int rism_run_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads,
+                   rism_thermo_out *out, int *status, int *steps);
+
 /* Mixed-precision contexts (opts.precision = 1; no counterpart in the
  * reference): moves the loop state between the FP64 level (prec 0) and the
  * FP32 level (prec 1), keeping the accelerator's history.  rism_solve moves

@@ -1,0 +1,98 @@
+"""Batched solves (rism_solve_batch / rism_run_batch; no counterpart in the
+reference, which runs one solute per process): many contexts of uneven grid
+sizes advanced together, each on its own stream without a host round trip
+per step.  The batch must give exactly the results of separate solves done
+step by step (the same kernels in the same order), including the step at
+which each one stops, whatever its outcome (converged, diverged, step limit).
+"""
+import numpy as np
+import pytest
+
+import gen_inputs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _solvent(rism, golden):
+    return rism.read_xvk(f"{golden}/h2o_vv.xvk")
+
+
+def _solute(sol):
+    xyz = sol["xyz"]
+    mn, mx = xyz.min(0), xyz.max(0)
+    shift = np.array([np.floor(-(mx[i] - mn[i]) / 2 - mn[i] + 0.5) for i in range(3)])
+    return {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"], "uxyz": xyz + shift}
+
+
+def _case(i):
+    sol = G.batch_solute(i)
+    grid, box = G.batch_grid(sol["xyz"], sizes=(48, 60, 64), pad=8.0)
+    return grid, box, _solute(sol)
+
+
+def _stepwise(rism, solv, grid, box, su, closure, maxstep, tol):
+    """rism_run with per-kernel timing on: the step-by-step loop that reads the
+    rms back after every step."""
+    g = rism.RISM3D(grid, box, solv, su, closure=closure)
+    g.kernel_timing(True)
+    th, tm = g.run(maxstep=maxstep, tol=tol)
+    t = g.fields(rism.FIELD_T)
+    g.close()
+    return th, tm, t
+
+
+def test_batch_equals_stepwise(rism, gpu, golden):
+    solv = _solvent(rism, golden)
+    cases = [_case(i) for i in range(6)]
+    print([c[0] for c in cases])
+    ref = [_stepwise(rism, solv, g, b, su, rism.KH, 1000, 1e-6) for g, b, su in cases]
+    gs = [rism.RISM3D(g, b, solv, su) for g, b, su in cases]
+    out = rism.run_batch(gs, maxstep=1000, tol=1e-6, threads=3)
+    for (th, st, steps), g, (thr, tmr, tr) in zip(out, gs, ref):
+        assert st == tmr["status"] == rism.STATUS_CONVERGED
+        assert steps == tmr["steps"]
+        for k in ("SFE_SC", "SFE_GF", "PMV"):
+            assert th[k] == thr[k], k
+        assert np.array_equal(g.fields(rism.FIELD_T), tr)
+        g.close()
+
+
+def test_batch_mixed_outcomes(rism, gpu, golden):
+    """A converging KH solve, an HNC solve that overflows (divergence) and one
+    stopped by the step limit, in one batch."""
+    solv = _solvent(rism, golden)
+    g0, b0, su0 = _case(0)
+    # the 24-atom solute whose HNC solve overflows exp(-u + t) (test_gpu_solver)
+    strong = _solute(G.cfg1_solute())
+    specs = [(g0, b0, su0, rism.KH, 1000), ((48, 48, 48), (24.0, 24.0, 24.0), strong, rism.HNC, 1000),
+             (g0, b0, su0, rism.KH, 7)]
+    ref = [_stepwise(rism, solv, g, b, su, cl, ms, 1e-6) for g, b, su, cl, ms in specs]
+    assert ref[1][1]["status"] == rism.STATUS_DIVERGED
+    assert ref[2][1]["status"] == rism.STATUS_MAXSTEP
+    for (g, b, su, cl, ms), (thr, tmr, tr) in zip(specs, ref):
+        h = rism.RISM3D(g, b, solv, su, closure=cl)
+        (th, st, steps), = rism.run_batch([h], maxstep=ms, tol=1e-6)
+        assert (st, steps) == (tmr["status"], tmr["steps"])
+        if st != rism.STATUS_DIVERGED:
+            assert th["SFE_SC"] == thr["SFE_SC"]
+            assert np.array_equal(h.fields(rism.FIELD_T), tr)
+        h.close()
+    # all three together
+    hs = [rism.RISM3D(g, b, solv, su, closure=cl) for g, b, su, cl, ms in specs]
+    for h in hs:
+        h.initialize()
+    res = rism.solve_batch(hs[:2], maxstep=1000, tol=1e-6, threads=2)
+    assert [tuple(r[:2]) for r in res] == [(t["status"], t["steps"]) for _, t, _ in ref[:2]]
+    for h in hs:
+        h.close()
+
+
+def test_batch_rejects_unsupported(rism, gpu, golden):
+    solv = _solvent(rism, golden)
+    g, b, su = _case(1)
+    h = rism.RISM3D(g, b, solv, su, precision=1)
+    h.initialize()
+    with pytest.raises(rism.RismError) as e:
+        rism.solve_batch([h])
+    assert e.value.code == rism.RISM_ERR_ARG
+    h.close()
