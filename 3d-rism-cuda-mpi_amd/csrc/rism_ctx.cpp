@@ -154,6 +154,7 @@ struct rism_ctx {
   double temper = 0, xt = 0;
   int natu = 0;
   std::vector<double> atoms_host;  // per atom record, rism::atom_stride(natv) doubles
+  std::vector<double> usig, ueps, uq, uxyz;  // the solute as given (for .huv/.cuv headers)
   rism_opts opts{};
   // device: the FP64 level, and the FP32 level of a mixed-precision context
   Level<double> d;
@@ -191,6 +192,19 @@ struct rism_ctx {
   int *dflag = nullptr, *hflag = nullptr;
   size_t rms_cap = 0;
   bool async = false;
+  // one period of mixing steps captured as a graph (see async_enqueue), with
+  // the state it was captured from
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  struct GraphKey {
+    int iT, iR, s0, s1, b;
+    double tol;
+    const double *hist;
+    bool operator==(const GraphKey &o) const {
+      return iT == o.iT && iR == o.iR && s0 == o.s0 && s1 == o.s1 && b == o.b && tol == o.tol &&
+             hist == o.hist;
+    }
+  } gkey{};
   bool have_step = false;
   rism_timing timing{};
   // per-kernel hipEvent timing (rism_set_kernel_timing)
@@ -218,6 +232,8 @@ struct rism_ctx {
     return RISM_OK;
   }
   ~rism_ctx() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (hscal) (void)hipHostFree(hscal);
     if (hflag) (void)hipHostFree(hflag);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
@@ -588,7 +604,7 @@ int an2_step(rism_ctx *c) {
 // In async mode (AN2 only) nothing is read back: the finish kernel records
 // rms_hist[step] and raises the done flag, which stops every later kernel.
 template <class Re>
-int calc_step(rism_ctx *c, double *rms, bool lin = false, int step = 0, double tol = -1.0) {
+int calc_step(rism_ctx *c, double *rms, bool lin = false, double tol = -1.0) {
   Level<Re> &L = lv<Re>(c);
   rism::Params P = c->P;
   P.mode = lin ? 3 : 0;
@@ -634,7 +650,7 @@ int calc_step(rism_ctx *c, double *rms, bool lin = false, int step = 0, double t
     KTime kt(c, RISM_K_REDUCE);
     HIPCHECK(rism::run_reduce_cols(c->partial, nb, W, c->natv, c->sums, P.done, c->st));
     if (!andm)
-      HIPCHECK(rism::run_step_finish(c->sums, c->rho_dev, c->natv, c->N, tol, step,
+      HIPCHECK(rism::run_step_finish(c->sums, c->rho_dev, c->natv, c->N, tol,
                                      c->count == 0, c->opts.an2_mp, c->binary, c->scal,
                                      c->async ? c->rms_hist : nullptr,
                                      c->async ? c->dflag : nullptr, c->st));
@@ -851,7 +867,7 @@ int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, b
 // flag once per chunk.  The arithmetic is that of rism_solve's step-by-step
 // loop, so the results are identical; only the iteration count is found
 // afterwards, from the flag.
-constexpr int kAsyncChunk = 16;  // steps enqueued between reads of the flag
+constexpr int kAsyncChunk = 24;  // steps enqueued between reads of the flag
 
 struct AsyncRun {
   rism_ctx *c = nullptr;
@@ -883,21 +899,88 @@ int async_begin(AsyncRun &a, rism_ctx *c, int maxstep, double tol) {
     if (rc) return rc;
     c->rms_cap = cap;
   }
-  HIPCHECK(hipMemsetAsync(c->dflag, 0, 2 * sizeof(int), c->st));
+  HIPCHECK(hipMemsetAsync(c->dflag, 0, 3 * sizeof(int), c->st));
   c->async = true;
   return RISM_OK;
 }
 
-// enqueues up to `chunk` steps and the read-back of the flag
+// The buffer indices of the AN2 slots repeat every kGraphSteps mixing steps
+// (t ping-pong x the rotation of three residual buffers), and no kernel
+// argument depends on the step number, so one period is captured once per
+// context (and tolerance) as a hipGraph and replayed.
+constexpr int kGraphSteps = 6;
+
+// the host-side index bookkeeping of one AN2 update (an2_step) without
+// launching it
+void advance_indices(rism_ctx *c) {
+  const int b = c->binary;
+  const int oldR = c->iR_slot[b];
+  if (c->count > 0) --c->count;
+  c->iR_slot[b] = c->iR_cur;
+  c->iR_cur = oldR;
+  c->iT_cur = 1 - c->iT_cur;
+  c->binary = (b == 0) ? 1 : 0;
+}
+
+int ensure_graph(rism_ctx *c, double tol) {
+  const rism_ctx::GraphKey key{c->iT_cur, c->iR_cur, c->iR_slot[0], c->iR_slot[1], c->binary,
+                               tol, c->rms_hist};
+  if (c->gexec && key == c->gkey) return RISM_OK;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  // capture from the current state, then restore it (capture runs nothing)
+  const int sT = c->iT_cur, sR = c->iR_cur, s0 = c->iR_slot[0], s1 = c->iR_slot[1],
+            sb = c->binary, scount = c->count;
+  HIPCHECK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  int rc = RISM_OK;
+  for (int k = 0; k < kGraphSteps && !rc; ++k) {
+    rc = calc_step<double>(c, nullptr, false, tol);
+    if (!rc) rc = an2_step<double>(c);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->st, &g);
+  c->iT_cur = sT;
+  c->iR_cur = sR;
+  c->iR_slot[0] = s0;
+  c->iR_slot[1] = s1;
+  c->binary = sb;
+  c->count = scount;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return rism::set_error(RISM_ERR_HIP, "hipStreamEndCapture failed");
+  c->graph = g;
+  HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  c->gkey = key;
+  return RISM_OK;
+}
+
+// enqueues up to `chunk` steps and the read-back of the flag: Picard steps
+// and remainders directly, whole periods of mixing steps as graph launches
 int async_enqueue(AsyncRun &a, int chunk) {
   rism_ctx *c = a.c;
   int rc;
-  for (int k = 0; k < chunk && a.istep < a.maxstep; ++k) {
+  for (int k = 0; k < chunk && a.istep < a.maxstep;) {
+    if (c->count == 0 && k + kGraphSteps <= chunk && a.istep + kGraphSteps <= a.maxstep) {
+      if ((rc = ensure_graph(c, a.tol))) return rc;
+      HIPCHECK(hipGraphLaunch(c->gexec, c->st));
+      for (int j = 0; j < kGraphSteps; ++j) {
+        const int s = ++a.istep;
+        a.log[s] = AsyncRun::Log{c->iT_cur, c->d.R[c->iR_cur]};
+        advance_indices(c);
+      }
+      k += kGraphSteps;
+      continue;
+    }
     const int s = ++a.istep;
     a.log[s] = AsyncRun::Log{c->iT_cur, c->d.R[c->iR_cur]};
-    if ((rc = calc_step<double>(c, nullptr, false, s, a.tol))) return rc;
+    if ((rc = calc_step<double>(c, nullptr, false, a.tol))) return rc;
     c->fresh = false;
     if ((rc = an2_step<double>(c))) return rc;
+    ++k;
   }
   HIPCHECK(hipMemcpyAsync(c->hflag, c->dflag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
   return RISM_OK;
@@ -1084,6 +1167,10 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   c->sigv.assign(solvent->sig, solvent->sig + c->natv);
   c->epsv.assign(solvent->eps, solvent->eps + c->natv);
   c->natu = solute->natu;
+  c->usig.assign(solute->sig, solute->sig + c->natu);
+  c->ueps.assign(solute->eps, solute->eps + c->natu);
+  c->uq.assign(solute->q, solute->q + c->natu);
+  c->uxyz.assign(solute->xyz, solute->xyz + 3 * (size_t)c->natu);
   c->P.clos = opts->closure;
   fill_params(c);
 
@@ -1137,10 +1224,10 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   if ((rc = c->alloc(&c->scratch, c->N))) return fail(rc);
   c->hsums.assign(kSumWidth * c->natv, 0.0);
   if ((rc = c->alloc(&c->scal, 8))) return fail(rc);
-  if ((rc = c->alloc(&c->dflag, 2))) return fail(rc);
+  if ((rc = c->alloc(&c->dflag, 3))) return fail(rc);
   if ((rc = upload(c, &c->rho_dev, c->rho))) return fail(rc);
   if (hipHostMalloc((void **)&c->hscal, 8 * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&c->hflag, 2 * sizeof(int)) != hipSuccess)
+      hipHostMalloc((void **)&c->hflag, 3 * sizeof(int)) != hipSuccess)
     return fail(rism::set_error(RISM_ERR_NOMEM, "hipHostMalloc failed"));
   reset_an2(c);
   if (hipMemsetAsync(c->d.T[0], 0, sizeof(double) * nn, c->st) != hipSuccess ||
@@ -1516,6 +1603,47 @@ int rism_write_guv(rism_ctx *c, const char *path, const double shift[3]) {
   }
   return o ? RISM_OK : rism::set_error(RISM_ERR_IO, std::string("write failed: ") + path);
 }
+
+// .huv / .cuv text dumps (output_huv.cu:7-50, output_cuv.cu:7-50, one rank):
+// box, grid, natu, per atom "q sig eps x y z", natv, then one line per grid
+// point with the natv values, default stream formatting.  The .cuv values
+// keep the reference's "(re,im)" form (it writes the complex h minus t).
+// The reference's host h array is never filled from the device, so its
+// files hold zeros; these hold the fields of the last step.
+static int write_text_field(rism_ctx *c, const char *path, int which) {
+  if (!c || !path) return rism::set_error(RISM_ERR_ARG, "null argument");
+  if (!c->have_step) return rism::set_error(RISM_ERR_STATE, "no step taken yet");
+  std::vector<double> f((size_t)c->natv * c->N);
+  for (int v = 0; v < c->natv; ++v) {
+    int rc = rism_get_field(c, which, v, f.data() + (size_t)v * c->N);
+    if (rc) return rc;
+  }
+  std::ofstream o(path);
+  if (!o) return rism::set_error(RISM_ERR_IO, std::string("cannot write ") + path);
+  o << c->box[0] << " " << c->box[1] << " " << c->box[2] << " " << std::endl
+    << c->nx << " " << c->ny << " " << c->nz << " " << std::endl
+    << c->natu << std::endl;
+  for (int a = 0; a < c->natu; ++a)
+    o << c->uq[a] << " " << c->usig[a] << " " << c->ueps[a] << " " << c->uxyz[3 * a] << " "
+      << c->uxyz[3 * a + 1] << " " << c->uxyz[3 * a + 2] << std::endl;
+  o << c->natv << std::endl;
+  for (size_t i = 0; i < c->N; ++i) {
+    for (int v = 0; v < c->natv; ++v) {
+      const double x = f[(size_t)v * c->N + i];
+      if (which == RISM_FIELD_C)
+        o << "(" << x << ",0) ";
+      else
+        o << x << " ";
+    }
+    o << "\n";
+  }
+  o.flush();
+  return o ? RISM_OK : rism::set_error(RISM_ERR_IO, std::string("write failed: ") + path);
+}
+
+int rism_write_huv(rism_ctx *c, const char *path) { return write_text_field(c, path, RISM_FIELD_H); }
+
+int rism_write_cuv(rism_ctx *c, const char *path) { return write_text_field(c, path, RISM_FIELD_C); }
 
 int rism_write_tuv(rism_ctx *c, const char *path) {
   if (!c || !path) return rism::set_error(RISM_ERR_ARG, "null argument");

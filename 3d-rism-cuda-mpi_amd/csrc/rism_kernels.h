@@ -97,12 +97,15 @@ hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, 
                            const int *done, hipStream_t st);
 // End of one AN2 step on the device (cal_rms.cu:20-24 and, when the next
 // update mixes, cal_theta + leg of anderson.cu:81-130): from the per-site
-// sums (6 per site) it writes scal = {rms, x0, x1, s1, s2}, rms_hist[step]
-// (if non-null) and, when rms <= tol or rms is not finite, done = {1 or 2,
-// step}.  binary: the slot the next update retires.
+// sums (6 per site) it writes scal = {rms, x0, x1, s1, s2}.  With a flag
+// array done = {state, step, count} it also counts the step (count + 1),
+// writes rms_hist[step] and, when rms <= tol or rms is not finite, sets
+// state to 1 or 2 and step; no argument depends on the step number, so a
+// run of steps can be captured once and replayed.  binary: the slot the
+// next update retires.
 hipError_t run_step_finish(const double *sums, const double *rho, int natv, size_t N,
-                           double tol, int step, int mixing, double mp, int binary,
-                           double *scal, double *rms_hist, int *done, hipStream_t st);
+                           double tol, int mixing, double mp, int binary, double *scal,
+                           double *rms_hist, int *done, hipStream_t st);
 template <class Re>
 hipError_t run_an2_picard(Re *tnew, const Re *t, const Re *R, Re *Pb, double m, size_t n,
                           const int *done, hipStream_t st);

@@ -104,11 +104,17 @@ __global__ void k_an2_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restri
 // elimination without pivoting anderson.cu:105-130, the mp shift
 // anderson.cu:66-67).  One thread; the sums are 6 doubles per site.
 __global__ void k_step_finish(const double *__restrict__ sums, const double *__restrict__ rho,
-                              int natv, size_t N, double tol, int step, int mixing, double mp,
-                              int binary, double *__restrict__ scal, double *__restrict__ rms_hist,
+                              int natv, size_t N, double tol, int mixing, double mp, int binary,
+                              double *__restrict__ scal, double *__restrict__ rms_hist,
                               int *__restrict__ done) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (done && done[0]) return;
+  // step number: done[2] counts the finished steps of the run
+  int step = 0;
+  if (done) {
+    step = done[2] + 1;
+    done[2] = step;
+  }
   double ss = 0.0, cc0 = 0.0, cc1 = 0.0, a0 = 0.0, a1 = 0.0, a3 = 0.0;
   for (int v = 0; v < natv; ++v) {
     const double *s = sums + 6 * v;
@@ -491,10 +497,10 @@ hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, 
 }
 
 hipError_t run_step_finish(const double *sums, const double *rho, int natv, size_t N,
-                           double tol, int step, int mixing, double mp, int binary,
-                           double *scal, double *rms_hist, int *done, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(64), 0, st, sums, rho, natv, N, tol, step,
-                     mixing, mp, binary, scal, rms_hist, done);
+                           double tol, int mixing, double mp, int binary, double *scal,
+                           double *rms_hist, int *done, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(64), 0, st, sums, rho, natv, N, tol, mixing,
+                     mp, binary, scal, rms_hist, done);
   return hipGetLastError();
 }
 

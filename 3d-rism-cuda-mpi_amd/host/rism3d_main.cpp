@@ -160,13 +160,25 @@ int main(int argc, char *argv[]) {
     const std::string fxmu = fname + (q0 ? ".xmu0" : ".xmu");
     if ((rc = rism_write_xmu(fxmu.c_str(), in->closure, &th)) != RISM_OK) return die("write_xmu");
   }
+  if (outlist.find('c') != std::string::npos) {  // output_cuv.cu:9-17 (rank 0)
+    const std::string f = fname + ".cuv";
+    std::cout << "outputting Cuv to file:  " << f << "  ..." << std::endl;
+    if ((rc = rism_write_cuv(ctx, (f + "0").c_str())) != RISM_OK) return die("write_cuv");
+    std::cout << "done." << std::endl;
+  }
   if (outlist.find('g') != std::string::npos) {
     const std::string fguv = fname + ".guv";
     std::cout << "outputting Guv to file:  " << fguv << "  ..." << std::endl;
     if ((rc = rism_write_guv(ctx, fguv.c_str(), in->shift)) != RISM_OK) return die("write_guv");
     std::cout << "done." << std::endl;
   }
-  for (char f : std::string("dchab"))
+  if (outlist.find('h') != std::string::npos) {  // output_huv.cu:9-17 (rank 0)
+    const std::string f = fname + ".huv";
+    std::cout << "outputting Huv to file:  " << f << "  ..." << std::endl;
+    if ((rc = rism_write_huv(ctx, (f + "0").c_str())) != RISM_OK) return die("write_huv");
+    std::cout << "done." << std::endl;
+  }
+  for (char f : std::string("dab"))
     if (outlist.find(f) != std::string::npos)
       std::cout << "output '" << f << "' is not produced by this build" << std::endl;
   rism_destroy(ctx);

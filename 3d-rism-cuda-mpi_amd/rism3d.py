@@ -41,7 +41,7 @@ EXPORTS = [
     "rism_write_guv", "rism_write_tuv", "rism_read_tuv", "rism_k_fft_r2c",
     "rism_k_fft_c2r", "rism_k_get_fk", "rism_k_an2_state", "rism_set_kernel_timing",
     "rism_get_kernel_stats", "rism_set_precision", "rism_get_precision",
-    "rism_solve_batch", "rism_run_batch",
+    "rism_solve_batch", "rism_run_batch", "rism_write_huv", "rism_write_cuv",
 ]
 KERNELS = ["zfwd_closure", "ypass_fwd", "xpass_oz", "ypass_inv", "zinv_resid", "reduce",
            "an2_update", "potential_r", "fk_half", "thermo_partial"]
@@ -165,6 +165,8 @@ def lib():
     L.rism_write_xmu.argtypes = [ctypes.c_char_p, ci, ctypes.POINTER(Thermo)]
     L.rism_write_guv.argtypes = [vp, ctypes.c_char_p, _dp]
     L.rism_write_tuv.argtypes = [vp, ctypes.c_char_p]
+    L.rism_write_huv.argtypes = [vp, ctypes.c_char_p]
+    L.rism_write_cuv.argtypes = [vp, ctypes.c_char_p]
     L.rism_read_tuv.argtypes = [vp, ctypes.c_char_p]
     L.rism_k_fft_r2c.argtypes = [ci, ci, ci, _dp, _dp, _dp]
     L.rism_k_fft_c2r.argtypes = [ci, ci, ci, _dp, _dp, _dp]
@@ -450,6 +452,14 @@ class RISM3D:
 
     def write_tuv(self, path):
         _check(lib().rism_write_tuv(self._ctx, str(path).encode()))
+
+    def write_huv(self, path):
+        """output_huv: text dump of h (last step) with the solute header."""
+        _check(lib().rism_write_huv(self._ctx, str(path).encode()))
+
+    def write_cuv(self, path):
+        """output_cuv: text dump of c = h - t as "(re,im)" pairs."""
+        _check(lib().rism_write_cuv(self._ctx, str(path).encode()))
 
     def read_tuv(self, path):
         _check(lib().rism_read_tuv(self._ctx, str(path).encode()))

@@ -284,6 +284,12 @@ int rism_write_xmu(const char *path, int closure, const rism_thermo_out *th);
 int rism_write_guv(rism_ctx *ctx, const char *path, const double shift[3]);
 /* Writes the text .tuv checkpoint of write_tuv.cu:19-28. */
 int rism_write_tuv(rism_ctx *ctx, const char *path);
+/* Replace RISM3D::output_huv / output_cuv (output_huv.cu:7-50,
+ * output_cuv.cu:7-50; the CLI names the files <base>.huv0 / <base>.cuv0 as
+ * the reference's rank 0 does): text dumps of h and c = h - t of the last
+ * step with the solute header; .cuv values in the reference's "(re,im)" form. */
+int rism_write_huv(rism_ctx *ctx, const char *path);
+int rism_write_cuv(rism_ctx *ctx, const char *path);
 /* Reads a .tuv checkpoint and uploads it as t (read_tuv.cu:6-29, with the
  * H2D copy the reference omits). */
 int rism_read_tuv(rism_ctx *ctx, const char *path);

@@ -20,6 +20,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -87,3 +88,41 @@ def test_cli_mobley(rism, oracle, gpu, golden, tmp_path):
     assert abs(got["PMV"] - ref["PMV"]) <= 1e-5
     text = open(tmp_path / "mobley_3053621.xmu").read().splitlines()
     assert text[0] == " $RESULTS" and text[-1] == " $END"
+
+
+def _read_text_dump(path):
+    with open(path) as f:
+        box = [float(x) for x in f.readline().split()]
+        grid = [int(x) for x in f.readline().split()]
+        natu = int(f.readline())
+        atoms = [[float(x) for x in f.readline().split()] for _ in range(natu)]
+        natv = int(f.readline())
+        rows = [line.replace("(", " ").replace(",0)", " ").split() for line in f]
+    vals = np.array([[float(x) for x in r] for r in rows])
+    return box, grid, np.array(atoms), natv, vals
+
+
+def test_huv_cuv_dumps(rism, oracle, gpu, golden, tmp_path):
+    """.huv / .cuv (output_huv.cu, output_cuv.cu): header with box, grid and
+    the solute, then one line per grid point (x fastest) with h, resp. c as
+    "(re,im)", to the stream's 6 significant digits."""
+    d = rism.read_input(os.path.join(golden, "3drism.dat"),
+                        os.path.join(golden, "mobley_3053621"), solvent_dir=golden)
+    g = rism.RISM3D((32, 32, 32), (32.0, 32.0, 32.0), d, d)
+    g.input = d
+    g.initialize()
+    g.iterate(maxstep=3, tol=1e-6)
+    g.write_huv(tmp_path / "m.huv0")
+    g.write_cuv(tmp_path / "m.cuv0")
+    h = g.fields(rism.FIELD_H).reshape(g.natv, -1)
+    c = g.fields(rism.FIELD_C).reshape(g.natv, -1)
+    for name, ref in (("m.huv0", h), ("m.cuv0", c)):
+        box, grid, atoms, natv, vals = _read_text_dump(tmp_path / name)
+        assert box == g.box and grid == g.grid and natv == g.natv
+        assert atoms.shape == (len(g.input["uq"]), 6)
+        assert np.allclose(atoms[:, 0], g.input["uq"], rtol=1e-5)
+        assert vals.shape == (g.N, g.natv)
+        assert np.allclose(vals.T, ref, rtol=1e-5, atol=1e-6 * np.abs(ref).max())
+    text = open(tmp_path / "m.cuv0").read().splitlines()
+    assert text[-1].strip().startswith("(") and text[-1].strip().endswith(",0)")
+    g.close()
