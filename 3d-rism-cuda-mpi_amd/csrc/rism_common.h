@@ -88,6 +88,26 @@ constexpr int tile_cols() {
   return b;
 }
 
+// Tiles of the z passes, which read and write real fields with the z-plane
+// stride: for M >= 128 and FP64 a tile row is 256 B of contiguous x (32
+// columns) within ~148 KB of LDS (one workgroup per CU, 1024 threads); FP32
+// keeps 32 columns (128 B), which measured faster than 64 columns with two
+// butterflies per thread; smaller columns keep the tiles of the y pass.
+template <int M, class Re>
+struct ZTile {
+  static constexpr int CS = FftShape<M>::CS;
+  static constexpr int wide() {
+    int b = 32;
+    while (b > 1 && (long)b * CS * (long)sizeof(cx<Re>) > 150 * 1024) b /= 2;
+    return b;
+  }
+  static constexpr int B = (M >= 128) ? wide() : tile_cols<M, Re>();
+  static constexpr int T = (B * FftShape<M>::T > 1024) ? 1024 / B : FftShape<M>::T;
+  // the residual pass with the Anderson(m) sums needs more registers: half
+  // as many columns (512 threads)
+  static constexpr int BH = (M >= 128 && B > 1) ? B / 2 : B;
+};
+
 static inline unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Workgroups of a 1D grid are handed to the 8 XCDs of the MI355X round-robin

@@ -645,7 +645,7 @@ int calc_step(rism_ctx *c, double *rms, bool lin = false, double tol = -1.0) {
                               r0, c->partial, L.tb.twz, L.tb.zpre, L.tb.zpost, use_hist, c->st));
     }
   }
-  const size_t nb = rism::zinv_num_blocks<Re>(c->P) / c->natv;
+  const size_t nb = rism::zinv_num_blocks<Re>(c->P, andm ? 1 : 0) / c->natv;
   {
     KTime kt(c, RISM_K_REDUCE);
     HIPCHECK(rism::run_reduce_cols(c->partial, nb, W, c->natv, c->sums, P.done, c->st));
@@ -1216,7 +1216,10 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   }
   if (c->opts.anderson_m > 0)
     c->D.assign((size_t)(c->opts.anderson_m + 1) * (c->opts.anderson_m + 1), 0.0);
-  c->zinv_blocks = rism::zinv_num_blocks<double>(c->P);
+  c->zinv_blocks = std::max(std::max(rism::zinv_num_blocks<double>(c->P, 0),
+                                     rism::zinv_num_blocks<double>(c->P, 1)),
+                            std::max(rism::zinv_num_blocks<float>(c->P, 0),
+                                     rism::zinv_num_blocks<float>(c->P, 1)));
   if ((rc = c->alloc(&c->partial, c->zinv_blocks * kSumWidth))) return fail(rc);
   const int tb = rism::thermo_blocks(c->N);
   if ((rc = c->alloc(&c->tpartial, (size_t)tb * 4 * c->natv))) return fail(rc);
@@ -1753,7 +1756,7 @@ int rism_k_fft_c2r(int nx, int ny, int nz, const double box[3], const double *in
   cplx *spec;
   double *partial;
   if ((rc = c->alloc(&h, c->N)) || (rc = c->alloc(&spec, c->N / 2)) ||
-      (rc = c->alloc(&partial, rism::zinv_num_blocks<double>(c->P) * 6)))
+      (rc = c->alloc(&partial, rism::zinv_num_blocks<double>(c->P, 0) * 6)))
     return rc;
   HIPCHECK(hipMemcpy(spec, in, sizeof(cplx) * (c->N / 2), hipMemcpyHostToDevice));
   rism::Params P = c->P;

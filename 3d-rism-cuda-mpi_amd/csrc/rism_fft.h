@@ -274,11 +274,13 @@ __device__ __forceinline__ void stockham_run(cx<Re> *lds, int tid, const cx<Re> 
 // TO_GLOBAL: the last stage calls store(i, v); otherwise the result is left in
 // LDS and the function ends with a barrier.  S = -1 forward, +1 inverse
 // (unnormalised).
-template <int L, int S, bool FROM_GLOBAL, bool TO_GLOBAL, class Re, class Load, class Store>
+// TT threads work on the column (default FftShape<L>::T, one largest-radix
+// butterfly each; fewer threads take several butterflies per stage).
+template <int L, int S, bool FROM_GLOBAL, bool TO_GLOBAL, int TT = FftShape<L>::T, class Re,
+          class Load, class Store>
 __device__ __forceinline__ void fft_column(cx<Re> *lds, int tid, const cx<Re> *__restrict__ tw,
                                            Load &load, Store &store) {
-  constexpr int T = FftShape<L>::T;
-  stockham_run<L, S, T, 0, 1, FROM_GLOBAL, TO_GLOBAL>(lds, tid, tw, load, store);
+  stockham_run<L, S, TT, 0, 1, FROM_GLOBAL, TO_GLOBAL>(lds, tid, tw, load, store);
 }
 
 }  // namespace rism

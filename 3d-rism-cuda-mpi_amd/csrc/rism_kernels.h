@@ -54,8 +54,9 @@ struct HistUpdateT {
 inline __host__ __device__ int atom_stride(int natv) { return 5 + 3 * natv; }
 
 bool fft_length_supported(int L);
+// workgroups of the residual pass (hm: with the Anderson(m) sums)
 template <class Re>
-size_t zinv_num_blocks(const Params &P);
+size_t zinv_num_blocks(const Params &P, int hm);
 int xoz_threads(int L, int natv);
 int thermo_blocks(size_t N);
 

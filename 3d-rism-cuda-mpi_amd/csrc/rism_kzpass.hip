@@ -14,14 +14,14 @@ namespace rism {
 // M-point DFT, G(m) = E(m) + w_m O(m) with E, O from Z(m) and Z(M-1-m) and
 // w_m = e^{-2 pi i (m+1/2)/nz}: the z-DFT at the positive half-shifted
 // frequencies.
-template <int M, int B, class Re>
-__global__ void __launch_bounds__(B *FftShape<M>::T)
+template <int M, int B, int TZ, class Re>
+__global__ void __launch_bounds__(B *TZ)
     k_zfwd_closure(const Re *__restrict__ t, const Re *__restrict__ u,
                    const Re *__restrict__ fr, Params P, cx<Re> *__restrict__ spec,
                    const cx<Re> *__restrict__ twM, const cx<Re> *__restrict__ zpre,
                    const cx<Re> *__restrict__ zpost) {
   using C = cx<Re>;
-  constexpr int T = FftShape<M>::T, CS = FftShape<M>::CS;
+  constexpr int T = TZ, CS = FftShape<M>::CS;
   __shared__ C lds[B * CS];
   if (P.done && *P.done) return;
   const int tid = threadIdx.x;
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
     return cmul(C{cval(2 * p), cval(2 * p + 1)}, zpre[p]);
   };
   auto store = [&](int, C) {};
-  fft_column<M, -1, true, false>(col, j, twM, load, store);
+  fft_column<M, -1, true, false, TZ>(col, j, twM, load, store);
 
   C *out = spec + ((size_t)v * M * P.ny + y) * P.nx + x;
   const size_t mstride = (size_t)P.ny * P.nx;
@@ -86,8 +86,8 @@ struct ZinvSums {
   static constexpr int W = HM == 0 ? 6 : 1 + RISM_MAX_HIST;
 };
 
-template <int M, int B, int HM, class Re>
-__global__ void __launch_bounds__(B *FftShape<M>::T)
+template <int M, int B, int TZ, int HM, class Re>
+__global__ void __launch_bounds__(B *TZ)
     k_zinv_resid(const cx<Re> *__restrict__ spec, const Re *__restrict__ t,
                  const Re *__restrict__ u, Re *__restrict__ R, const Re *__restrict__ R0,
                  const Re *__restrict__ R1, const Re *__restrict__ r0, HistT<Re> H, Params P,
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
                  const cx<Re> *__restrict__ twM, const cx<Re> *__restrict__ zpre,
                  const cx<Re> *__restrict__ zpost, int use_hist) {
   using C = cx<Re>;
-  constexpr int T = FftShape<M>::T, CS = FftShape<M>::CS;
+  constexpr int T = TZ, CS = FftShape<M>::CS;
   constexpr int NT = B * T;
   constexpr int W = ZinvSums<HM>::W;
   // the LDS tile doubles as the reduction buffer: W doubles per wave, or per
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(B *FftShape<M>::T)
     resid(2 * p, zz.x);
     resid(2 * p + 1, zz.y);
   };
-  fft_column<M, +1, false, true>(col, j, twM, load, store);
+  fft_column<M, +1, false, true, TZ>(col, j, twM, load, store);
 
   // workgroup reduction of the W sums: wave shuffles then LDS when the
   // workgroup is made of whole waves, LDS only otherwise (tiny grids)
@@ -223,18 +223,18 @@ template <int M, class Re>
 static void launch_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<Re> *spec,
                         const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost,
                         hipStream_t st) {
-  constexpr int B = tile_cols<M, Re>();
-  hipLaunchKernelGGL((k_zfwd_closure<M, B, Re>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T), 0,
-                     st, t, u, fr, P, spec, twM, zpre, zpost);
+  constexpr int B = ZTile<M, Re>::B, TZ = ZTile<M, Re>::T;
+  hipLaunchKernelGGL((k_zfwd_closure<M, B, TZ, Re>), dim3(zgrid(P, B)), dim3(B * TZ), 0, st, t, u,
+                     fr, P, spec, twM, zpre, zpost);
 }
 template <int M, int HM, class Re>
 static void launch_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
                         const Re *R0, const Re *R1, const Re *r0, const HistT<Re> &H,
                         double *partial, const cx<Re> *twM, const cx<Re> *zpre,
                         const cx<Re> *zpost, int use_hist, hipStream_t st) {
-  constexpr int B = tile_cols<M, Re>();
-  hipLaunchKernelGGL((k_zinv_resid<M, B, HM, Re>), dim3(zgrid(P, B)), dim3(B * FftShape<M>::T),
-                     0, st, spec, t, u, R, R0, R1, r0, H, P, partial, twM, zpre, zpost, use_hist);
+  constexpr int B = HM ? ZTile<M, Re>::BH : ZTile<M, Re>::B, TZ = ZTile<M, Re>::T;
+  hipLaunchKernelGGL((k_zinv_resid<M, B, TZ, HM, Re>), dim3(zgrid(P, B)), dim3(B * TZ), 0, st, spec,
+                     t, u, R, R0, R1, r0, H, P, partial, twM, zpre, zpost, use_hist);
 }
 bool fft_length_supported(int L) {
   switch (L) {
@@ -249,13 +249,13 @@ bool fft_length_supported(int L) {
 // Workgroups of one z pass; the same for both bulk types (the FP32 tiles are
 // twice as wide, so pass the type the solver runs in).
 template <class Re>
-size_t zinv_num_blocks(const Params &P) {
+size_t zinv_num_blocks(const Params &P, int hm) {
   const int M = P.nz / 2;
   int B = 16;
   switch (M) {
-#define X(n)               \
-  case n:                  \
-    B = tile_cols<n, Re>(); \
+#define X(n)                                         \
+  case n:                                            \
+    B = hm ? ZTile<n, Re>::BH : ZTile<n, Re>::B;     \
     break;
     RISM_LENGTHS(X)
 #undef X
@@ -319,7 +319,7 @@ hipError_t run_zinv_hist(const Params &P, const cx<Re> *spec, const Re *t, const
 }
 
 #define INST(Re)                                                                              \
-  template size_t zinv_num_blocks<Re>(const Params &);                                       \
+  template size_t zinv_num_blocks<Re>(const Params &, int);                                       \
   template hipError_t run_zfwd<Re>(const Params &, const Re *, const Re *, const Re *,       \
                                    cx<Re> *, const cx<Re> *, const cx<Re> *, const cx<Re> *, \
                                    hipStream_t);                                             \
