@@ -1646,6 +1646,79 @@ static int write_text_field(rism_ctx *c, const char *path, int which) {
 
 int rism_write_huv(rism_ctx *c, const char *path) { return write_text_field(c, path, RISM_FIELD_H); }
 
+int rism_analysis(rism_ctx *c, int kind, double *out) {
+  if (!c || !out) return rism::set_error(RISM_ERR_ARG, "null argument");
+  if (kind < RISM_ANALYSIS_GRAD || kind > RISM_ANALYSIS_AD)
+    return rism::set_error(RISM_ERR_ARG, "unknown analysis kind");
+  if (!c->have_step || c->last_prec != 0)
+    return rism::set_error(RISM_ERR_STATE, "the analysis needs a solved FP64 state");
+  const int nv = c->natv, nu = c->natu;
+  if (nu == 0) return RISM_OK;
+  // cal_LJ.cu:33-44: sigma_uv (lambda-scaled for -l), eps_uv = sqrt(eps_u eps_v)
+  const double lam1 = (c->opts.ad_switch == 1) ? c->opts.lambda : 1.0;
+  std::vector<double> adata((size_t)nu * (4 + 2 * nv));
+  for (int a = 0; a < nu; ++a) {
+    double *p = adata.data() + (size_t)a * (4 + 2 * nv);
+    p[0] = c->uxyz[3 * a];
+    p[1] = c->uxyz[3 * a + 1];
+    p[2] = c->uxyz[3 * a + 2];
+    p[3] = c->uq[a];
+    for (int v = 0; v < nv; ++v) {
+      p[4 + v] = (c->usig[a] + c->sigv[v]) * 0.5 * lam1;
+      p[4 + nv + v] = std::sqrt(c->ueps[a] * c->epsv[v]);
+    }
+  }
+  const int k = kind == RISM_ANALYSIS_GRAD ? 0
+              : kind == RISM_ANALYSIS_EUV  ? 1
+              : (c->opts.ad_switch == 1 ? 2 : 3);
+  const int width = k == 0 ? 3 : (k == 1 ? nv : 1);
+  const size_t nslab = (c->N + rism::atom_sum_slab(c->N) - 1) / rism::atom_sum_slab(c->N);
+  double *dad = nullptr, *dpart = nullptr, *dout = nullptr;
+  auto cleanup = [&]() {
+    if (dad) (void)hipFree(dad);
+    if (dpart) (void)hipFree(dpart);
+    if (dout) (void)hipFree(dout);
+  };
+  if (hipMalloc(&dad, sizeof(double) * adata.size()) != hipSuccess ||
+      hipMalloc(&dpart, sizeof(double) * nslab * nu * width) != hipSuccess ||
+      hipMalloc(&dout, sizeof(double) * nu * width) != hipSuccess) {
+    (void)hipGetLastError();
+    cleanup();
+    return rism::set_error(RISM_ERR_NOMEM, "hipMalloc failed in rism_analysis");
+  }
+  std::vector<double> sums((size_t)nu * width);
+  const double cc = kHartree * kBohr * kAvogadro;  // cal_grad.cu:47
+  hipError_t e = hipMemcpyAsync(dad, adata.data(), sizeof(double) * adata.size(),
+                                hipMemcpyHostToDevice, c->st);
+  if (e == hipSuccess)
+    e = rism::run_atom_sums(k, c->d.T[c->iT_last], c->d.u, c->P, dad, nu,
+                            c->opts.lambda, cc, c->rho_dev, dpart, c->st);
+  if (e == hipSuccess)
+    e = rism::run_reduce_cols(dpart, nslab, nu * width, 1, dout, nullptr, c->st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sums.data(), dout, sizeof(double) * sums.size(), hipMemcpyDeviceToHost,
+                       c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  cleanup();
+  if (e != hipSuccess) return rism::set_error(RISM_ERR_HIP, hipGetErrorString(e));
+  const double dv = c->P.dv;
+  constexpr double kcal2J = 4184.0;  // physical.h:17
+  if (k == 1) {  // output_euv.cu:13-27: dv * (total, per site), rho_v-weighted
+    for (int a = 0; a < nu; ++a) {
+      double tot = 0.0;
+      for (int v = 0; v < nv; ++v) {
+        const double ev = sums[(size_t)a * nv + v] * c->rho[v];
+        out[(size_t)a * (nv + 1) + 1 + v] = dv * ev;
+        tot += ev;
+      }
+      out[(size_t)a * (nv + 1)] = dv * tot;
+    }
+  } else {  // output_grad.cu:13-20, output_ad.cu:15-19: * dv / kcal2J
+    for (size_t i = 0; i < sums.size(); ++i) out[i] = sums[i] * dv / kcal2J;
+  }
+  return RISM_OK;
+}
+
 int rism_write_cuv(rism_ctx *c, const char *path) { return write_text_field(c, path, RISM_FIELD_C); }
 
 int rism_write_tuv(rism_ctx *c, const char *path) {

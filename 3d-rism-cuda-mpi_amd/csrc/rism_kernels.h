@@ -134,5 +134,12 @@ template <class Re>
 hipError_t run_thermo_partial(const Re *t, const Re *th, const Re *u, const Re *R,
                               const Params &P, size_t N, double *partial, hipStream_t st);
 hipError_t run_apply_phase(cplx *s, const Params &P, int inverse, hipStream_t st);
+// per-atom integrals of g (kind 0 grad, 1 euv, 2 ad1, 3 ad2); adata: per atom
+// x y z q sigma_uv[natv] eps_uv[natv]; partial: [slab][natu][width] with
+// width 3, natv, 1, 1 and atom_sum_slab(N) points per slab
+size_t atom_sum_slab(size_t N);
+hipError_t run_atom_sums(int kind, const double *t, const double *u, const Params &P,
+                         const double *adata, int natu, double lambda, double cc,
+                         const double *rho, double *partial, hipStream_t st);
 
 }  // namespace rism

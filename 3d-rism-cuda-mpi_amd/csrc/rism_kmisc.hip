@@ -326,6 +326,121 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---------------------------------------------------------------------------
+// per-atom integrals of the converged g (cal_grad.cu:44-90, cal_euv.cu:30-70,
+// cal_ad1.cu:40-80, cal_ad2.cu:40-75): for each solute atom u, sums over the
+// grid points r with |r - r_u| >= sigma_uv / 2 (sigma_uv lambda / 2 for ad1)
+// of g_v(r) times
+//   kind 0 (grad):  [24 eps s6 (2 s6 - 1) / r^2 + cc q_u q_v / r^3] (r - r_u)
+//   kind 1 (euv):   4 eps s6 (s6 - 1) + cc q_u q_v / r          (per site v)
+//   kind 2 (ad1):   24 eps s6 (2 s6 - 1) / lambda
+//   kind 3 (ad2):   cc q_u q_v / r
+// with s6 = (sigma_uv^2 / r^2)^3, eps = sqrt(eps_u eps_v) in J/mol, cc =
+// E_h a_0 N_A; kinds 0, 2, 3 weight each site by rho_v.  One thread per atom
+// (atoms in registers), grid points with their g values staged in LDS; each
+// workgroup covers one slab of points and writes its sums per atom.
+// ---------------------------------------------------------------------------
+constexpr int AS_TILE = 256;
+
+template <int KIND>
+__global__ void __launch_bounds__(256)
+    k_atom_sums(const double *__restrict__ t, const double *__restrict__ u, Params P,
+                const double *__restrict__ adata, int natu, double lambda, double cc,
+                const double *__restrict__ rho, size_t slab, int width,
+                double *__restrict__ partial) {
+  __shared__ double sp[AS_TILE * (3 + RISM_MAX_SITES / 2)];
+  const int nv = P.natv;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = a < natu;
+  // atom record: x y z q, then sigma_uv[nv], eps_uv[nv], q_v folded below
+  const double *ad = adata + (size_t)(active ? a : 0) * (4 + 2 * nv);
+  const double ax = ad[0], ay = ad[1], az = ad[2], aq = ad[3];
+  double acc[KIND == 0 ? 3 : (KIND == 1 ? 8 : 1)];
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(acc) / sizeof(double)); ++k) acc[k] = 0.0;
+  const size_t N = (size_t)P.nx * P.ny * P.nz;
+  const size_t p0 = (size_t)blockIdx.y * slab;
+  const size_t p1 = p0 + slab < N ? p0 + slab : N;
+  double *px = sp, *py = sp + AS_TILE, *pz = sp + 2 * AS_TILE, *pg = sp + 3 * AS_TILE;
+  for (size_t q0 = p0; q0 < p1; q0 += AS_TILE) {
+    const int nq = (int)((p1 - q0) < (size_t)AS_TILE ? (p1 - q0) : (size_t)AS_TILE);
+    __syncthreads();
+    for (int k = threadIdx.x; k < nq; k += blockDim.x) {
+      const size_t i = q0 + k;
+      const int x = (int)(i % P.nx), y = (int)((i / P.nx) % P.ny), z = (int)(i / ((size_t)P.nx * P.ny));
+      px[k] = (double)(x - P.nx / 2) * P.dr[0];
+      py[k] = (double)(y - P.ny / 2) * P.dr[1];
+      pz[k] = (double)(z - P.nz / 2) * P.dr[2];
+    }
+    // g of every site for the tile, AS_TILE doubles per site (in two halves
+    // of the LDS when nv > 4)
+    for (int vh = 0; vh < nv; vh += 4) {
+      const int nvh = nv - vh < 4 ? nv - vh : 4;
+      for (int k = threadIdx.x; k < nq * nvh; k += blockDim.x) {
+        const int w = k / nq, kk = k % nq;
+        const size_t i = (size_t)(vh + w) * N + q0 + kk;
+        pg[w * AS_TILE + kk] = closure_g(P.clos, t[i], u[i]);
+      }
+      __syncthreads();
+      if (active) {
+        for (int w = 0; w < nvh; ++w) {
+          const int v = vh + w;
+          const double sg = ad[4 + v], eps = ad[4 + nv + v];
+          const double cut = (KIND == 2 ? sg * lambda : sg) * 0.5;
+          const double cut2 = cut * cut;
+          const double sg2 = sg * sg;
+          const double qq = cc * aq * P.qv[v];
+          const double wv = (KIND == 1) ? 1.0 : rho[v];
+          double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+          for (int k = 0; k < nq; ++k) {
+            const double dx = px[k] - ax, dy = py[k] - ay, dz = pz[k] - az;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < cut2) continue;  // r < sigma / 2 (r = 0 included)
+            const double g = pg[w * AS_TILE + k];
+            const double ir = rsq_nr(r2);
+            const double ir2 = ir * ir;
+            const double rs2 = sg2 * ir2, rs6 = rs2 * rs2 * rs2;
+            if (KIND == 0) {
+              const double f = (eps * 24.0 * rs6 * ir2 * (2.0 * rs6 - 1.0) + qq * ir2 * ir) * g;
+              s0 += f * dx;
+              s1 += f * dy;
+              s2 += f * dz;
+            } else if (KIND == 1) {
+              s0 += (eps * 4.0 * rs6 * (rs6 - 1.0) + qq * ir) * g;
+            } else if (KIND == 2) {
+              s0 += eps * 24.0 * rs6 * (2.0 * rs6 - 1.0) / lambda * g;
+            } else {
+              s0 += qq * ir * g;
+            }
+          }
+          if (KIND == 0) {
+            acc[0] += wv * s0;
+            acc[1] += wv * s1;
+            acc[2] += wv * s2;
+          } else if (KIND == 1) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+              if (k == v) acc[k] += s0;
+          } else {
+            acc[0] += wv * s0;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (!active) return;
+  double *out = partial + ((size_t)blockIdx.y * natu + a) * width;
+  if (KIND == 1) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < nv) out[k] = acc[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(acc) / sizeof(double)); ++k) out[k] = acc[k];
+  }
+}
+
 // fk on the half spectrum (cal_Coulomb.cu:65-79, 90-95), stored raw:
 // fkraw = conj(phi(k)) fk, phi(k) = -i (-1)^(nx/2 + ny/2 + jx + jy + m).
 // Each thread owns FKP consecutive jx of one (jy, m) row; e^{-i kx x_u} is
@@ -626,6 +741,40 @@ hipError_t run_thermo_partial(const Re *t, const Re *th, const Re *u,
                               hipStream_t st) {
   dim3 grid(thermo_blocks(N), P.natv);
   hipLaunchKernelGGL(k_thermo_partial<Re>, grid, dim3(256), 0, st, t, th, u, R, P, N, partial);
+  return hipGetLastError();
+}
+
+size_t atom_sum_slab(size_t N) {
+  size_t s = N / 1024;
+  if (s < 4096) s = 4096;
+  return (s + AS_TILE - 1) / AS_TILE * AS_TILE;
+}
+
+hipError_t run_atom_sums(int kind, const double *t, const double *u, const Params &P,
+                         const double *adata, int natu, double lambda, double cc,
+                         const double *rho, double *partial, hipStream_t st) {
+  const size_t N = (size_t)P.nx * P.ny * P.nz;
+  const size_t slab = atom_sum_slab(N);
+  const int width = kind == 0 ? 3 : (kind == 1 ? P.natv : 1);
+  dim3 grid((natu + 255) / 256, (unsigned)((N + slab - 1) / slab));
+  switch (kind) {
+    case 0:
+      hipLaunchKernelGGL(k_atom_sums<0>, grid, dim3(256), 0, st, t, u, P, adata, natu, lambda,
+                         cc, rho, slab, width, partial);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_atom_sums<1>, grid, dim3(256), 0, st, t, u, P, adata, natu, lambda,
+                         cc, rho, slab, width, partial);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_atom_sums<2>, grid, dim3(256), 0, st, t, u, P, adata, natu, lambda,
+                         cc, rho, slab, width, partial);
+      break;
+    default:
+      hipLaunchKernelGGL(k_atom_sums<3>, grid, dim3(256), 0, st, t, u, P, adata, natu, lambda,
+                         cc, rho, slab, width, partial);
+      break;
+  }
   return hipGetLastError();
 }
 

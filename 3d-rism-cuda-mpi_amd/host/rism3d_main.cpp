@@ -14,7 +14,9 @@
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 #include <string>
+#include <vector>
 
 #include "../../include/rism3d_c.h"
 
@@ -160,6 +162,16 @@ int main(int argc, char *argv[]) {
     const std::string fxmu = fname + (q0 ? ".xmu0" : ".xmu");
     if ((rc = rism_write_xmu(fxmu.c_str(), in->closure, &th)) != RISM_OK) return die("write_xmu");
   }
+  if (outlist.find('d') != std::string::npos) {  // cal_grad + output_grad.cu:8-22
+    const std::string f = fname + ".gra";
+    std::vector<double> du((size_t)in->natu * 3);
+    if ((rc = rism_analysis(ctx, RISM_ANALYSIS_GRAD, du.data())) != RISM_OK) return die("grad");
+    std::cout << "outputting grad to file:  " << f << "  ..." << std::endl;
+    std::ofstream o(f);
+    for (int a = 0; a < in->natu; ++a)
+      o << du[3 * a] << " " << du[3 * a + 1] << " " << du[3 * a + 2] << std::endl;
+    std::cout << "done." << std::endl;
+  }
   if (outlist.find('c') != std::string::npos) {  // output_cuv.cu:9-17 (rank 0)
     const std::string f = fname + ".cuv";
     std::cout << "outputting Cuv to file:  " << f << "  ..." << std::endl;
@@ -178,9 +190,30 @@ int main(int argc, char *argv[]) {
     if ((rc = rism_write_huv(ctx, (f + "0").c_str())) != RISM_OK) return die("write_huv");
     std::cout << "done." << std::endl;
   }
-  for (char f : std::string("dab"))
-    if (outlist.find(f) != std::string::npos)
-      std::cout << "output '" << f << "' is not produced by this build" << std::endl;
+  if (outlist.find('a') != std::string::npos) {  // cal_ad1/cal_ad2 + output_ad.cu:8-22
+    std::stringstream ss;
+    ss << fname << "-" << adswitch << "-" << lambda;
+    std::vector<double> du(in->natu);
+    if ((rc = rism_analysis(ctx, RISM_ANALYSIS_AD, du.data())) != RISM_OK) return die("ad");
+    std::cout << "outputting ad to file:  " << ss.str() << "  ..." << std::endl;
+    std::ofstream o(ss.str());
+    for (int a = 0; a < in->natu; ++a) o << du[a] << std::endl;
+    std::cout << "done." << std::endl;
+  }
+  if (outlist.find('b') != std::string::npos) {  // cal_euv + output_euv.cu:8-31
+    const std::string f = fname + ".euv";
+    const int w = 1 + in->natv;
+    std::vector<double> e((size_t)in->natu * w);
+    if ((rc = rism_analysis(ctx, RISM_ANALYSIS_EUV, e.data())) != RISM_OK) return die("euv");
+    std::cout << "outputting euv to file:  " << f << "  ..." << std::endl;
+    std::ofstream o(f);
+    for (int a = 0; a < in->natu; ++a) {
+      o << std::fixed << e[(size_t)a * w];
+      for (int v = 1; v < w; ++v) o << " " << std::fixed << e[(size_t)a * w + v];
+      o << std::endl;
+    }
+    std::cout << "done." << std::endl;
+  }
   rism_destroy(ctx);
   rism_free_input(in);
   return 0;

@@ -24,6 +24,7 @@ RISM_ERR_ARG, RISM_ERR_IO, RISM_ERR_VERSION, RISM_ERR_CLOSURE = 1, 2, 3, 4
 RISM_ERR_NATOM, RISM_ERR_KRANGE, RISM_ERR_HIP, RISM_ERR_NOMEM, RISM_ERR_STATE = 5, 6, 7, 8, 9
 STATUS_CONVERGED, STATUS_DIVERGED, STATUS_MAXSTEP = 0, 1, 2
 KH, HNC = 0, 1
+ANALYSIS_GRAD, ANALYSIS_EUV, ANALYSIS_AD = 0, 1, 2
 
 
 def PSE(n):
@@ -42,6 +43,7 @@ EXPORTS = [
     "rism_k_fft_c2r", "rism_k_get_fk", "rism_k_an2_state", "rism_set_kernel_timing",
     "rism_get_kernel_stats", "rism_set_precision", "rism_get_precision",
     "rism_solve_batch", "rism_run_batch", "rism_write_huv", "rism_write_cuv",
+    "rism_analysis",
 ]
 KERNELS = ["zfwd_closure", "ypass_fwd", "xpass_oz", "ypass_inv", "zinv_resid", "reduce",
            "an2_update", "potential_r", "fk_half", "thermo_partial"]
@@ -166,6 +168,7 @@ def lib():
     L.rism_write_guv.argtypes = [vp, ctypes.c_char_p, _dp]
     L.rism_write_tuv.argtypes = [vp, ctypes.c_char_p]
     L.rism_write_huv.argtypes = [vp, ctypes.c_char_p]
+    L.rism_analysis.argtypes = [vp, ci, _dp]
     L.rism_write_cuv.argtypes = [vp, ctypes.c_char_p]
     L.rism_read_tuv.argtypes = [vp, ctypes.c_char_p]
     L.rism_k_fft_r2c.argtypes = [ci, ci, ci, _dp, _dp, _dp]
@@ -328,6 +331,7 @@ class RISM3D:
         us.append(_f64(np.asarray(solute["uxyz"]).reshape(-1)))
         self._keep += us
         su = SoluteC(natu, *[_ptr(a) for a in us])
+        self._natu = natu
         g = Grid(self.grid[0], self.grid[1], self.grid[2], (ctypes.c_double * 3)(*self.box))
         op = Opts(int(closure), int(count), float(m), float(mp), int(ad_switch), float(lam),
                   int(bool(verbose)), int(bool(chi_spline)), int(anderson_m), int(precision),
@@ -452,6 +456,16 @@ class RISM3D:
 
     def write_tuv(self, path):
         _check(lib().rism_write_tuv(self._ctx, str(path).encode()))
+
+    def analysis(self, kind):
+        """Per-atom integrals of g (cal_grad / cal_euv / cal_ad): kind 0 grad
+        (natu x 3, kcal/mol/A), 1 euv (natu x (1 + natv), J/mol), 2 ad
+        (natu, kcal/mol), as written to .gra / .euv / <base>-<s>-<lambda>."""
+        n = self._natu
+        width = {0: 3, 1: 1 + self.natv, 2: 1}[int(kind)]
+        out = np.zeros(n * width)
+        _check(lib().rism_analysis(self._ctx, int(kind), _ptr(out)))
+        return out.reshape(n, width) if width > 1 else out
 
     def write_huv(self, path):
         """output_huv: text dump of h (last step) with the solute header."""

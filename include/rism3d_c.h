@@ -289,6 +289,17 @@ int rism_write_tuv(rism_ctx *ctx, const char *path);
  * the reference's rank 0 does): text dumps of h and c = h - t of the last
  * step with the solute header; .cuv values in the reference's "(re,im)" form. */
 int rism_write_huv(rism_ctx *ctx, const char *path);
+
+/* ---- analysis outputs (output.cu:52-105) -------------------------------- */
+#define RISM_ANALYSIS_GRAD 0 /* cal_grad + output_grad: natu x 3, kcal/mol/A  */
+#define RISM_ANALYSIS_EUV 1  /* cal_euv + output_euv: natu x (1 + natv), J/mol:
+                                the atom's total, then per solvent site      */
+#define RISM_ANALYSIS_AD 2   /* cal_ad1 (-l) or cal_ad2 + output_ad: natu,
+                                kcal/mol                                    */
+/* Per-atom integrals of the g of the last FP64 step (cal_grad.cu,
+ * cal_euv.cu, cal_ad1.cu, cal_ad2.cu), scaled as the reference writes them
+ * to .gra / .euv / <base>-<switch>-<lambda>. */
+int rism_analysis(rism_ctx *ctx, int kind, double *out);
 int rism_write_cuv(rism_ctx *ctx, const char *path);
 /* Reads a .tuv checkpoint and uploads it as t (read_tuv.cu:6-29, with the
  * H2D copy the reference omits). */

@@ -497,6 +497,56 @@ def thermo(solv: Solvent, dv, clos, h, t, u=None):
     }
 
 
+def atom_sums(kind, grid, box, solv: Solvent, sol: Solute, g, lam=1.0, adswitch=0):
+    """Per-atom integrals of g (cal_grad.cu:44-90, cal_euv.cu:30-70,
+    cal_ad1.cu:40-80, cal_ad2.cu:40-75), scaled as output_grad.cu,
+    output_euv.cu and output_ad.cu write them.  g: [natv][N] on the grid.
+    kind 0 grad -> (natu, 3) kcal/mol/A; 1 euv -> (natu, 1 + natv) J/mol
+    (total, then per site); 2 ad -> (natu,) kcal/mol (cal_ad1 when
+    adswitch == 1, else cal_ad2).  Brute force over all points."""
+    nx, ny, nz = (int(v) for v in grid)
+    dr = np.asarray(box, dtype=float) / np.array([nx, ny, nz])
+    dv = float(np.prod(box)) / (nx * ny * nz)
+    cc = HARTREE * BOHR * AVOGADRO
+    lam1 = lam if adswitch == 1 else 1.0
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    rx = ((x - nx // 2) * dr[0]).reshape(-1)
+    ry = ((y - ny // 2) * dr[1]).reshape(-1)
+    rz = ((z - nz // 2) * dr[2]).reshape(-1)
+    g = np.asarray(g).reshape(solv.natv, -1)
+    nu, nv = len(sol.q), solv.natv
+    out = []
+    for a in range(nu):
+        d = (rx - sol.xyz[a, 0], ry - sol.xyz[a, 1], rz - sol.xyz[a, 2])
+        r2 = d[0] ** 2 + d[1] ** 2 + d[2] ** 2
+        r1 = np.sqrt(r2)
+        acc = np.zeros(3 if kind == 0 else (nv if kind == 1 else 1))
+        for v in range(nv):
+            sg = (sol.sig[a] + solv.sig[v]) * 0.5 * lam1
+            eps = math.sqrt(sol.eps[a] * solv.eps[v])
+            qq = sol.q[a] * solv.q[v] * cc
+            cut = sg * (lam if (kind == 2 and adswitch == 1) else 1.0) * 0.5
+            m = r1 >= cut
+            with np.errstate(divide="ignore", invalid="ignore"):
+                rs6 = (sg * sg / r2[m]) ** 3
+                gv = g[v][m]
+                if kind == 0:
+                    f = (eps * 24.0 * rs6 / r2[m] * (2.0 * rs6 - 1.0) + qq / (r2[m] * r1[m])) * gv
+                    acc += solv.rho[v] * np.array([(f * d[i][m]).sum() for i in range(3)])
+                elif kind == 1:
+                    acc[v] += solv.rho[v] * ((eps * 4.0 * rs6 * (rs6 - 1.0) + qq / r1[m]) * gv).sum()
+                elif adswitch == 1:
+                    acc[0] += solv.rho[v] * (eps * 24.0 * rs6 * (2.0 * rs6 - 1.0) / lam * gv).sum()
+                else:
+                    acc[0] += solv.rho[v] * (qq / r1[m] * gv).sum()
+        if kind == 1:
+            out.append(np.concatenate([[dv * acc.sum()], dv * acc]))
+        else:
+            out.append(acc * dv / 4184.0)
+    out = np.array(out)
+    return out[:, 0] if kind == 2 else out
+
+
 def parse_xmu(path):
     """Reads the total lines of a .xmu file into a dict of floats."""
     out = {}

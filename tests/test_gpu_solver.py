@@ -217,3 +217,21 @@ def test_anderson_m_rejects_bad_history(rism, oracle, gpu, case24):
     with pytest.raises(rism.RismError) as e:
         rism.RISM3D(grid, box, d, _solute(sol), anderson_m=17)
     assert e.value.code == rism.RISM_ERR_ARG
+
+
+@pytest.mark.parametrize("ad,lam", [(0, 1.0), (1, 0.8), (2, 0.6)])
+def test_analysis_outputs(rism, oracle, gpu, case24, ad, lam):
+    """.gra / .euv / ad (cal_grad, cal_euv, cal_ad1, cal_ad2) against the
+    oracle's brute-force sums over the same g (the solved g of the context)."""
+    sol, s, d, grid, box = case24
+    g = rism.RISM3D(grid, box, d, _solute(sol), ad_switch=ad, lam=lam)
+    g.initialize()
+    g.iterate(maxstep=30, tol=1e-6)
+    gg = g.fields(rism.FIELD_G).reshape(s.natv, -1)
+    for kind in (rism.ANALYSIS_GRAD, rism.ANALYSIS_EUV, rism.ANALYSIS_AD):
+        got = g.analysis(kind)
+        ref = oracle.atom_sums(kind, grid, box, s, sol, gg, lam=lam, adswitch=ad)
+        scale = np.abs(ref).max()
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() < 1e-11 * scale, (kind, np.abs(got - ref).max(), scale)
+    g.close()
