@@ -125,6 +125,31 @@ __device__ __forceinline__ void oz_row(cx<Re> *lds, int CS, int jx, int L, const
   oz_dispatch<NVMAX, 2>(lds, CS, off, fk, P, X);
 }
 
+// The same contraction with the chi row already in registers (chr[v][w],
+// v, w < NV): used when every thread owns at most one (kx, -kx) pair, so the
+// row is fetched before the forward transform and its latency is hidden.
+template <int NV, class Re>
+__device__ __forceinline__ void oz_points_reg(cx<Re> *lds, int CS, const int (&off)[2],
+                                              const cx<Re> (&fk)[2], const Params &P,
+                                              const Re (&chr)[4][4]) {
+  cx<Re> c[2][NV];
+#pragma unroll
+  for (int w = 0; w < NV; ++w) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) c[k][w] = lds[off[k] + w * CS] - (Re)P.qv[w] * fk[k];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      cx<Re> a{0, 0};
+#pragma unroll
+      for (int w = 0; w < NV; ++w) a = a + chr[v][w] * c[k][w];
+      lds[off[k] + v * CS] = a;
+    }
+  }
+}
+
 // One workgroup per (m, y) row, all sites: natv columns of nx complex in LDS,
 // natv * T threads.  SHELL selects the chi source at compile time (the
 // cubic-box shell table or the spline at |k|) and NVMAX bounds the site
@@ -154,16 +179,51 @@ __global__ void __launch_bounds__(512)
   auto load = [&](int i) -> C { return cmul(row[i], prex[i]); };
   auto store = [&](int i, C vv) { row[i] = cmulc(vv, prex[i]); };
   auto nostore = [&](int, C) {};
-  fft_column<L, -1, true, false>(mylds, j, twL, load, nostore);
-
-  // OZ at every k of the row, the mirror points (kx, -kx) together
   const int nthr = nv * T;
   const int nv2 = nv * nv;
   // the difference-form step (mode 3): the long-range term cancels, fk = 0
   const C *fkr = (P.mode == 3) ? nullptr : fkraw + rowoff;
+  const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates
+  // at most one (kx, -kx) pair per thread and at most 4 sites: fetch this
+  // thread's chi row and fk pair before the forward transform
+  if constexpr (SHELL && NVMAX == 4) {
+    if (L / 2 <= nthr) {
+      Re chr[4][4];
+      C fkp[2] = {C{0, 0}, C{0, 0}};
+      const bool mine = tid < L / 2;
+      if (mine) {
+        const long ox = 2 * tid - L + 1;
+        const long sidx = (ox * ox + oy * oy + oz * oz - 3) >> 3;
+        const Re *rowp = chitab + sidx * nv2;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int w = 0; w < 4; ++w) chr[v][w] = (v < nv && w < nv) ? rowp[v * nv + w] : (Re)0;
+        if (fkr) {
+          fkp[0] = fkr[tid];
+          fkp[1] = fkr[L - 1 - tid];
+        }
+      }
+      fft_column<L, -1, true, false>(mylds, j, twL, load, nostore);
+      if (mine) {
+        const int off[2] = {lpad(tid), lpad(L - 1 - tid)};
+        switch (nv) {
+          case 1: oz_points_reg<1>(lds, CS, off, fkp, P, chr); break;
+          case 2: oz_points_reg<2>(lds, CS, off, fkp, P, chr); break;
+          case 3: oz_points_reg<3>(lds, CS, off, fkp, P, chr); break;
+          default: oz_points_reg<4>(lds, CS, off, fkp, P, chr); break;
+        }
+      }
+      __syncthreads();
+      fft_column<L, +1, false, true>(mylds, j, twL, load, store);
+      return;
+    }
+  }
+  fft_column<L, -1, true, false>(mylds, j, twL, load, nostore);
+
+  // OZ at every k of the row, the mirror points (kx, -kx) together
   for (int jx = tid; jx < L / 2; jx += nthr) {
     if constexpr (SHELL) {
-      const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates
       const long ox = 2 * jx - L + 1;
       const long sidx = (ox * ox + oy * oy + oz * oz - 3) >> 3;  // |k|^2 = (pi/L)^2 (8 sidx + 3)
       oz_row<NVMAX>(lds, CS, jx, L, fkr, P, ChiTable<Re>{chitab + sidx * nv2});
