@@ -45,15 +45,18 @@ __global__ void __launch_bounds__(B *TZ)
   const Re dv = (Re)P.dv;
   C *col = lds + c * CS;
 
+  // t and u are read once per step: non-temporal loads keep them out of the
+  // caches that fr (shared by the sites) and the twiddles use
   auto cval = [&](int z) -> Re {
     const size_t i = base + (size_t)z * plane;
-    const Re tt = tv[i];
+    const Re tt = __builtin_nontemporal_load(&tv[i]);
     // difference form of a correction sweep: t holds the correction d, u
     // the exponent a = -u + t0 at the base point, and c(t0 + d) - c(t0) =
     // g(a + d) - g(a) - d (the q fr term cancels)
-    if (P.mode == 3) return (closure_dg(clos, uv[i], tt) - tt) * dv;
+    const Re uu = __builtin_nontemporal_load(&uv[i]);
+    if (P.mode == 3) return (closure_dg(clos, uu, tt) - tt) * dv;
     if (P.mode != 0) return tt * dv;  // plain transform of t
-    const Re g = closure_g(clos, tt, uv[i]);
+    const Re g = closure_g(clos, tt, uu);
     return (g - (Re)1 - tt + q * fr[i]) * dv;  // trm1mt, times the forward dv
   };
   auto load = [&](int p) -> C {
@@ -154,15 +157,17 @@ __global__ void __launch_bounds__(B *TZ)
       return;
     }
     // difference form r0 + dh - dg, else tr (calculate_cuda.cu:58)
-    const Re rr = lin ? r0[voff + i] + h - closure_dg(clos, uv[i], tv[i])
-                      : h + (Re)1 - closure_g(clos, tv[i], uv[i]);
-    Rv[i] = rr;
+    const Re ti = __builtin_nontemporal_load(&tv[i]), ui = __builtin_nontemporal_load(&uv[i]);
+    const Re rr = lin ? r0[voff + i] + h - closure_dg(clos, ui, ti)
+                      : h + (Re)1 - closure_g(clos, ti, ui);
+    __builtin_nontemporal_store(rr, &Rv[i]);
     // the sums are FP64 whatever the bulk type
     const double r = (double)rr;
     s[0] += r * r;
     if constexpr (HM == 0) {
       if (use_hist) {
-        const double d0 = r - (double)R0v[i], d1 = r - (double)R1v[i];
+        const double d0 = r - (double)__builtin_nontemporal_load(&R0v[i]),
+                     d1 = r - (double)__builtin_nontemporal_load(&R1v[i]);
         s[1] += r * d0;
         s[2] += r * d1;
         s[3] += d0 * d0;
