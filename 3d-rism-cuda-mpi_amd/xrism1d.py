@@ -24,8 +24,11 @@ k = 0.
 Regenerating the bundled example/h2o_vv.xvk, naclaq_vv.xvk, kclaq_vv.xvk and
 example_milc/ctip3p.xvk from their own headers reproduces their chi(k) to
 2e-7 (1.3e-5 for ctip3p) and their compressibility to 2e-5 relative, which
-shows those files were made by XRISM-KH on the same grids; the dielectrically
-consistent (DRISM) variant is not provided.
+shows those files were made by XRISM-KH on the same grids.  With a target
+dielectric constant the solve is the dielectrically consistent DRISM
+(`dielectric=`); the dielectric constant of any solution follows from the
+k -> 0 charge sum rule (`dielectric_constant`): 1 + 3y for XRISM, the target
+for DRISM.
 """
 from __future__ import annotations
 
@@ -120,9 +123,83 @@ def _closure(name, e):
     raise ValueError(f"unknown closure {name}")
 
 
+def _drism_zeta(sites, k, ub, dielectric, a_smooth):
+    """The dielectric bridge of DRISM (Perkyns & Pettitt, Chem. Phys. Lett. 190,
+    626 (1992)) for the one polar species: zeta_ij(k) = f_i(k) f_j(k) h_c(k),
+    f_i = j0(k x_i) j0(k y_i) j1(k z_i) in the molecule's frame (centre of its
+    sites at the origin, dipole along z), h_c(k) = h_c0 exp(-k^2 a^2 / 4),
+    y = 4 pi beta rho mu^2 / 9.  Its k -> 0 part adds rho y h_c0 to the
+    dielectric constant XRISM gives (1 + 3 y), so h_c0 = ((eps - 1) / y - 3) /
+    rho makes the charge sum rule of the solution give eps itself (the form
+    (eps / y - 3) / rho realises eps + 1 under the same sum rule).  Returns
+    (zeta, rho) with zeta an (ns, ns, n) array."""
+    from scipy.special import spherical_jn
+
+    ns = len(sites)
+    zeta = np.zeros((ns, ns, len(k)))
+    polar = {}
+    for i, s in enumerate(sites):
+        polar.setdefault(s.species, []).append(i)
+    mus = {}
+    for spc, idx in polar.items():
+        if len(idx) < 2:
+            continue
+        xyz = np.array([sites[i].xyz for i in idx], dtype=float)
+        qv = np.array([sites[i].q for i in idx])
+        mu = (qv[:, None] * xyz).sum(0)
+        if np.linalg.norm(mu) > 1e-12:
+            mus[spc] = (idx, xyz, mu)
+    if len(mus) != 1:
+        raise ValueError("DRISM here supports exactly one polar species")
+    (spc, (idx, xyz, mu)), = mus.items()
+    rho = sites[idx[0]].density * AVOGADRO * 1e-27
+    # rotate the dipole onto +z
+    zhat = mu / np.linalg.norm(mu)
+    v = np.cross(zhat, [0.0, 0.0, 1.0])
+    c = float(np.dot(zhat, [0.0, 0.0, 1.0]))
+    if np.linalg.norm(v) < 1e-12:
+        rot = np.eye(3) if c > 0 else np.diag([1.0, -1.0, -1.0])
+    else:
+        vx = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
+        rot = np.eye(3) + vx + vx @ vx * (1.0 / (1.0 + c))
+    loc = (xyz - xyz.mean(0)) @ rot.T
+    mu2 = float(np.dot(mu, mu))
+    y = 4.0 * math.pi * ub * rho * mu2 / 9.0
+    hc = ((dielectric - 1.0) / y - 3.0) / rho * np.exp(-(k * a_smooth) ** 2 / 4.0)
+    f = [spherical_jn(0, k * p[0]) * spherical_jn(0, k * p[1]) * spherical_jn(1, k * p[2])
+         for p in loc]
+    for a_, i in enumerate(idx):
+        for b_, j in enumerate(idx):
+            zeta[i, j] = f[a_] * f[b_] * hc
+    return zeta, rho
+
+
+def dielectric_constant(res, sites, temper):
+    """eps from the k -> 0 limit of the charge structure factor, 1 - 1/eps =
+    (4 pi beta / k^2) sum_ab q_a q_b rho_a chi_ab(k), over the two smallest k
+    extrapolated quadratically (the merged chi summed over the sites a)."""
+    ub = HARTREE * BOHR / (BOLTZMANN * temper)
+    classes, mult, qc, rc = [], {}, {}, {}
+    for s in sites:
+        c = abs(s.sym)
+        if c not in classes:
+            classes.append(c)
+            qc[c], rc[c] = s.q, s.density * AVOGADRO * 1e-27
+        mult[c] = mult.get(c, 0) + 1
+    vals = []
+    for kk in (0, 1):
+        sqq = sum(mult[a] * rc[a] * qc[a] * qc[b] * res.chi[kk, i, j]
+                  for i, a in enumerate(classes) for j, b in enumerate(classes))
+        vals.append(4.0 * math.pi * ub * sqq / res.k[kk] ** 2)
+    x = vals[0] + (vals[0] - vals[1]) / 3.0
+    return 1.0 / (1.0 - x)
+
+
 def solve(sites, temper, dr=0.05, ntab=8192, closure="KH", tol=1e-11, eta=1.0, mix=0.3,
-          history=10, maxiter=3000, stages=(0.25, 0.5, 0.75, 1.0)):
-    """Solves XRISM for the given sites; returns a Result."""
+          history=10, maxiter=3000, stages=(0.25, 0.5, 0.75, 1.0), dielectric=None,
+          drism_a=1.0):
+    """Solves XRISM for the given sites (DRISM with `dielectric` = the target
+    dielectric constant); returns a Result."""
     g = _Grid(dr, ntab)
     ns, n = len(sites), g.n
     kT = BOLTZMANN * temper
@@ -153,7 +230,13 @@ def solve(sites, temper, dr=0.05, ntab=8192, closure="KH", tol=1e-11, eta=1.0, m
             lj[a, b] = 4.0 * e * (x6 * x6 - x6)
     short = ub * qq[:, :, None] * erfc(g.r / eta) / g.r
     longk = ub * qq[:, :, None] * 4.0 * math.pi / g.k ** 2 * np.exp(-(g.k * eta) ** 2 / 4.0)
-    Wk = np.moveaxis(W, -1, 0)
+    zeta = None
+    if dielectric is not None:
+        zeta, rho_p = _drism_zeta(sites, g.k, ub, dielectric, drism_a)
+        W2 = W + rho_p * zeta  # omega + rho zeta
+    else:
+        W2 = W
+    Wk = np.moveaxis(W2, -1, 0)
     eye = np.eye(ns)[None]
 
     def step(ts, scale):
@@ -162,8 +245,11 @@ def solve(sites, temper, dr=0.05, ntab=8192, closure="KH", tol=1e-11, eta=1.0, m
         Cs = g.fwd(cs)
         Ck = np.moveaxis(Cs - scale * longk, -1, 0)
         WC = Wk @ Ck
+        # (h - zeta) = W2 C W2 + W2 C P (h - zeta)  (DRISM; zeta = 0: XRISM)
         H = np.linalg.solve(eye - WC * rho[None, None, :], WC @ Wk)
         Hk = np.moveaxis(H, 0, -1)
+        if zeta is not None:
+            Hk = Hk + zeta
         return g.inv(Hk - Cs) - ts, Hk
 
     ts = np.zeros((ns, ns, n))
@@ -299,6 +385,8 @@ if __name__ == "__main__":
     ap.add_argument("--dr", type=float, default=0.05)
     ap.add_argument("--ntab", type=int, default=8192)
     ap.add_argument("--closure", default="KH")
+    ap.add_argument("--dielectric", type=float, default=None,
+                    help="DRISM with this dielectric constant (default: XRISM)")
     a = ap.parse_args()
     if a.like:
         regenerate(a.like, a.out, a.closure)
@@ -309,5 +397,5 @@ if __name__ == "__main__":
             sites += [ion("Na", 2, 3, 2.439, 0.0874, 1.0, a.molar), cl]
         elif a.salt == "KCl":
             sites += [ion("K", 2, 3, 3.038, 0.1937, 1.0, a.molar), cl]
-        res = solve(sites, a.temper, a.dr, a.ntab, a.closure)
+        res = solve(sites, a.temper, a.dr, a.ntab, a.closure, dielectric=a.dielectric)
         write_xvk(a.out, res, a.temper, a.dr)

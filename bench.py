@@ -3,8 +3,9 @@
 Workload (default "cfg2"): 512^3 grid in a 256 A box; a synthetic 12877-atom
 protein-like cluster (uniform in a 30 A sphere, >= 1.0 A apart, C/N/O/H + 1 %
 S, charges ~ N(0, 0.35 e) neutralised to +2 e, seed 1002); 0.2 M NaCl in
-TIP3P-type water (the bundled naclaq_vv.xvk chi table, 4 sites); KH closure;
-AN2 2 0.6 0.2; tolerance 1e-6; FP64 throughout.  One step = one full solve:
+TIP3P water (chi_vv from the project's 1D DRISM-KH at eps 78.4, 298.15 K,
+tests/golden/tip3p_nacl02_drism.xvk, 4 sites); KH closure; AN2 2 0.6 0.2;
+tolerance 1e-6; FP64 throughout.  One step = one full solve:
 LJ/Coulomb potentials, the initial t, the fixed-point loop to convergence and
 the thermodynamics, with the solute and the chi table already resident in
 HBM when the clock starts.
@@ -45,17 +46,18 @@ def workload(name):
     rism = ge.load_rism3d()
     if name == "cfg2":
         sol = G.cfg2_solute()
-        xvk = "naclaq_vv.xvk"
+        xvk = "tip3p_nacl02_drism.xvk"
         grid, box = (512, 512, 512), (256.0, 256.0, 256.0)
         desc = ("cfg2: 512^3 grid / 256 A box, synthetic 12877-atom cluster (R 30 A, +2 e, "
-                "seed 1002), 0.2 M NaCl TIP3P (bundled naclaq_vv.xvk chi, 4 sites), KH, "
-                "AN2 2 0.6 0.2, tol 1e-6, FP64")
+                "seed 1002), 0.2 M NaCl in TIP3P (chi from the project's 1D DRISM-KH, "
+                "eps 78.4, 298.15 K, 4 sites), KH, AN2 2 0.6 0.2, tol 1e-6, FP64")
     elif name == "cfg1":
         sol = G.cfg1_solute()
-        xvk = "h2o_vv.xvk"
+        xvk = "tip3p_drism.xvk"
         grid, box = (64, 64, 64), (32.0, 32.0, 32.0)
-        desc = ("cfg1: 64^3 / 32 A, 24-atom random-walk solute (seed 1001), water "
-                "(bundled h2o_vv.xvk chi), KH, AN2 2 0.6 0.2, tol 1e-6, FP64")
+        desc = ("cfg1: 64^3 / 32 A, 24-atom random-walk solute (seed 1001), TIP3P water "
+                "(chi from the project's 1D DRISM-KH, eps 78.4, 298.15 K), KH, AN2 2 0.6 0.2, "
+                "tol 1e-6, FP64")
     elif name == "mobley":
         d = rism.read_input(os.path.join(GOLDEN, "3drism.dat"),
                             os.path.join(GOLDEN, "mobley_3053621"), solvent_dir=GOLDEN)
@@ -160,7 +162,7 @@ def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv, budget_s=30.0):
 
 def bench_batch(args):
     """Batched small solutes: 256 random-walk solutes of 8-60 atoms (seeds
-    1004000 + i, net charge -1/0/+1) in water (bundled h2o_vv.xvk chi), each in
+    1004000 + i, net charge -1/0/+1) in TIP3P water (tip3p_drism.xvk), each in
     its own box of extent + 12 A padding per side at 0.5 A spacing, per-axis
     grid sizes from {48, 60, 64, 72, 80, 96}; KH, AN2 2 0.6 0.2, tol 1e-6,
     FP64.  Times the batch (rism_run_batch, one stream per solute, host
@@ -169,7 +171,7 @@ def bench_batch(args):
     import gen_inputs as G
     import __graft_entry__ as ge
     rism = ge.load_rism3d()
-    solv = rism.read_xvk(os.path.join(GOLDEN, "h2o_vv.xvk"))
+    solv = rism.read_xvk(os.path.join(GOLDEN, "tip3p_drism.xvk"))
     nsol = args.batch_size
     cases = []
     for i in range(nsol):
@@ -241,9 +243,9 @@ def bench_batch(args):
         "data": "synthetic",
         "config": {
             "workload": ("batch: %d random-walk solutes (8-60 atoms, seeds 1004000+i, net "
-                         "charge -1/0/+1) in water (bundled h2o_vv.xvk chi), boxes = extent + "
-                         "12 A/side at 0.5 A, grids from {48..96}^3, KH, AN2 2 0.6 0.2, "
-                         "tol 1e-6, FP64" % nsol),
+                         "charge -1/0/+1) in TIP3P water (chi from the project's 1D DRISM-KH, "
+                         "eps 78.4), boxes = extent + 12 A/side at 0.5 A, grids from "
+                         "{48..96}^3, KH, AN2 2 0.6 0.2, tol 1e-6, FP64" % nsol),
             "solves": nsol, "grid_shapes": len(set(c[0] for c in cases)),
             "batch_s": t_batch, "sequential_s": t_seq,
             "sequential_solves_per_s": nsol / t_seq,

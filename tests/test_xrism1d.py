@@ -67,3 +67,30 @@ def test_written_file_reads_back(rism, tmp_path):
     assert d["xt"] == pytest.approx(res.xt, rel=1e-7)
     assert np.allclose(np.asarray(d["xvv"]).reshape(res.chi.shape), res.chi, rtol=1e-14, atol=0)
     assert np.allclose(d["vq"], [-0.834, 0.417])
+
+
+def test_dielectric_constants(golden):
+    """The k -> 0 charge sum rule: XRISM water has eps = 1 + 3y (y = 4 pi beta
+    rho mu^2 / 9, the textbook XRISM result); DRISM reaches the requested
+    dielectric constant."""
+    sites, temper, dr, _ = X.read_header(os.path.join(golden, "h2o_vv.xvk"))
+    res = X.solve(sites, temper, dr, 2048)
+    rho = sites[0].density * X.AVOGADRO * 1e-27
+    mu = 2 * 0.417 * 0.58588
+    ub = X.HARTREE * X.BOHR / (X.BOLTZMANN * temper)
+    y = 4 * np.pi * ub * rho * mu ** 2 / 9
+    assert X.dielectric_constant(res, sites, temper) == pytest.approx(1 + 3 * y, rel=1e-4)
+    for eps in (78.4, 40.0):
+        r = X.solve(sites, temper, dr, 2048, dielectric=eps)
+        assert X.dielectric_constant(r, sites, temper) == pytest.approx(eps, rel=2e-4)
+
+
+@pytest.mark.slow
+def test_drism_salt_converges():
+    """DRISM with ions (0.2 M NaCl in TIP3P water, the salt of the 512^3
+    case): converges, and chi_vv(k) tends to w at large k."""
+    sites = X.water_tip3p(density=55.2) + [X.ion("Na", 2, 3, 2.439, 0.0874, 1.0, 0.2),
+                                           X.ion("Cl", 3, 4, 4.478, 0.0356, -1.0, 0.2)]
+    res = X.solve(sites, 298.15, 0.05, 4096, dielectric=78.4)
+    assert res.rms < 1e-10
+    assert np.allclose(res.chi[-1, 2, 2], 1.0, atol=1e-6)
