@@ -729,58 +729,99 @@ int convert_state(rism_ctx *c) {
   return RISM_OK;
 }
 
-// Restarts the accelerator at level Re: the reference's initial AN2 state
-// (Picard warm-up, zeroed slots) or an empty Anderson(m) history.
-template <class Re>
-int restart_accel(rism_ctx *c) {
-  Level<Re> &L = lv<Re>(c);
-  const size_t nn = c->N * c->natv;
-  c->count = c->opts.an2_count;
-  c->binary = 0;
-  c->iR_cur = 0;
-  c->iR_slot[0] = 1;
-  c->iR_slot[1] = 2;
-  c->hist.clear();
-  c->cur_rb = 0;
-  c->next_pb = 0;
-  if (c->opts.anderson_m == 0) {
-    for (int i = 0; i < 2; ++i) {
-      HIPCHECK(hipMemsetAsync(L.Pslot[i], 0, sizeof(Re) * nn, c->st));
-      HIPCHECK(hipMemsetAsync(L.R[c->iR_slot[i]], 0, sizeof(Re) * nn, c->st));
-    }
-  }
-  return RISM_OK;
-}
-
 template <class Re>
 int accel_step(rism_ctx *c) {
   c->fresh = false;
   return c->opts.anderson_m > 0 ? anderson_m_step<Re>(c) : an2_step<Re>(c);
 }
 
+// The accelerator's fields at level Re, slot for slot alike at both levels:
+// the mixed iterates (P = t + m R), which move with the base point of a
+// correction block, the history residuals, which do not, and the residual
+// of the current point.
+template <class Re>
+struct AccelFields {
+  std::vector<Re *> iter, resid;
+  Re *cur = nullptr;
+};
+template <class Re>
+AccelFields<Re> accel_fields(rism_ctx *c) {
+  Level<Re> &L = lv<Re>(c);
+  AccelFields<Re> a;
+  if (c->opts.anderson_m > 0) {
+    for (const auto &h : c->hist) {
+      a.iter.push_back(L.Pb[h.pb]);
+      a.resid.push_back(L.Rb[h.rb]);
+    }
+    a.cur = L.Rb[c->cur_rb];
+  } else {
+    for (int i = 0; i < 2; ++i) {
+      a.iter.push_back(L.Pslot[i]);
+      a.resid.push_back(L.R[c->iR_slot[i]]);
+    }
+    a.cur = L.R[c->iR_cur];
+  }
+  return a;
+}
+
+// FP64 state -> FP32 frame of a correction block based at t0 = d.T[k0], the
+// point the last FP64 step evaluated: the closure exponent and residual r0
+// at t0, d = 0, iterates relative to t0, residuals as they are.
+int rebase_down(rism_ctx *c, int k0) {
+  const size_t nn = c->N * c->natv;
+  const double *t0 = c->d.T[k0];
+  const AccelFields<double> D = accel_fields<double>(c);
+  const AccelFields<float> F = accel_fields<float>(c);
+  HIPCHECK(rism::run_exponent(t0, c->d.u, nn, c->f.u, c->st));
+  HIPCHECK(rism::run_convert((const double *)D.cur, F.cur, nn, c->st));
+  HIPCHECK(hipMemcpyAsync(c->f.aux, F.cur, sizeof(float) * nn, hipMemcpyDeviceToDevice, c->st));
+  for (size_t i = 0; i < D.iter.size(); ++i)
+    HIPCHECK(rism::run_rebase_down(D.iter[i], t0, F.iter[i], nn, c->st));
+  for (size_t i = 0; i < D.resid.size(); ++i)
+    HIPCHECK(rism::run_convert((const double *)D.resid[i], F.resid[i], nn, c->st));
+  HIPCHECK(hipMemsetAsync(c->f.T[k0], 0, sizeof(float) * nn, c->st));
+  c->prec = 1;
+  return RISM_OK;
+}
+
+// FP32 block frame -> FP64 state at t0 + d; t0 is overwritten last when the
+// current t shares its buffer.
+int rebase_up(rism_ctx *c, int k0) {
+  const size_t nn = c->N * c->natv;
+  const double *t0 = c->d.T[k0];
+  const AccelFields<double> D = accel_fields<double>(c);
+  const AccelFields<float> F = accel_fields<float>(c);
+  for (size_t i = 0; i < D.iter.size(); ++i)
+    HIPCHECK(rism::run_rebase_up(D.iter[i], t0, F.iter[i], nn, c->st));
+  for (size_t i = 0; i < D.resid.size(); ++i)
+    HIPCHECK(rism::run_convert((const float *)F.resid[i], D.resid[i], nn, c->st));
+  HIPCHECK(rism::run_rebase_up(c->d.T[c->iT_cur], t0, c->f.T[c->iT_cur], nn, c->st));
+  c->prec = c->last_prec = 0;
+  return RISM_OK;
+}
+
 // The mixed-precision solve (no counterpart in the reference).
 //  1. FP32 steps of the fixed-point map with the accelerator, while the FP32
 //     residual tracks the FP64 one: until rms <= hand_rms, or no progress
 //     for kStall steps.
-//  2. Defect-correction sweeps: one FP64 step at t0 gives the residual r0
-//     (and the convergence test); FP32 steps of the same map written in
-//     difference form, R(t0 + d) = r0 + [h(t0 + d) - h(t0)] -
-//     [g(t0 + d) - g(t0)], then solve for the correction d with the
-//     accelerator, and t0 += d.  The brackets are formed from d and the
-//     base exponent without cancellation, so the FP32 rounding scales with
-//     |d| and |r0|, not with |t|; no linearisation is involved.
-// Leaves the state at the FP64 level.  When a sweep stops gaining (or after
-// kMaxSweeps) it returns with *fallback set, and the caller continues with
-// plain FP64 steps.
+//  2. One accelerated iteration in which every (block + 1)-th residual is
+//     evaluated in FP64 and the others in FP32 in difference form about the
+//     last FP64 point t0: R(t0 + d) = r0 + [h(t0 + d) - h(t0)] -
+//     [g(t0 + d) - g(t0)], with r0 the FP64 residual at t0.  The brackets are
+//     formed from d and the base exponent without cancellation, so the FP32
+//     rounding scales with |d| (a block's worth of movement), not with |t|;
+//     no linearisation is involved.  The accelerator state moves between the
+//     two frames at every FP64 step (rebase_down / rebase_up), so its history
+//     is never discarded, and the convergence test is the FP64 residual.
+// When the FP64 residual stops falling for kNoGain blocks it returns with
+// *fallback set and the state at the FP64 level, and the caller continues
+// with plain FP64 steps from there.
 int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, bool &conver,
                 bool &diverge, int &low_steps, bool &fallback) {
   const bool verbose = c->opts.verbose != 0;
   const double hand_rms = std::max(tol, c->opts.mixed_switch > 0 ? c->opts.mixed_switch : 1e-3);
-  // a sweep reduces its residual by kEta at most: its FP32 rounding floor
-  // lies a few hundred times below the residual it starts from
-  constexpr int kStall = 25, kMaxSweeps = 12, kMaxInner = 600;
-  constexpr double kEta = 1e-2;
-  const size_t nn = c->N * c->natv;
+  const int block = c->opts.mixed_block > 0 ? c->opts.mixed_block : 24;
+  constexpr int kStall = 25, kNoGain = 3;
   int rc;
   // 1. FP32 iteration
   double best = HUGE_VAL;
@@ -800,16 +841,15 @@ int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, b
     } else {
       ++since_best;
     }
-    if (rms <= hand_rms || since_best >= kStall) break;
     if ((rc = accel_step<float>(c))) return rc;
+    if (rms <= hand_rms || since_best >= kStall) break;
   }
-  // the t of the last FP32 step becomes t0
-  HIPCHECK(rism::run_convert(c->f.T[c->iT_cur], c->d.T[c->iT_cur], nn, c->st));
-  c->prec = 0;
-  if ((rc = restart_accel<double>(c))) return rc;
-  // 2. correction sweeps
-  double prev = HUGE_VAL;
-  for (int sweep = 0; istep < maxstep; ++sweep) {
+  // the accelerator state of the last FP32 step, at the FP64 level
+  if ((rc = convert_state<float, double>(c))) return rc;
+  // 2. blocks of FP32 difference-form steps between FP64 steps
+  best = HUGE_VAL;
+  int no_gain = 0;
+  while (istep < maxstep) {
     ++istep;
     const int k0 = c->iT_cur;
     if ((rc = calc_step<double>(c, &rms))) return rc;
@@ -822,40 +862,29 @@ int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, b
       conver = true;
       return RISM_OK;
     }
-    if (sweep >= kMaxSweeps || rms > 0.5 * prev) {
+    if (rms < 0.95 * best) {
+      best = rms;
+      no_gain = 0;
+    } else if (++no_gain >= kNoGain) {
       fallback = true;
       return RISM_OK;
     }
-    prev = rms;
-    // the base point t0: its exponent and its residual, in FP32
-    HIPCHECK(rism::run_exponent(c->d.T[k0], c->d.u, nn, c->f.u, c->st));
-    HIPCHECK(rism::run_convert(c->d.Rlast, c->f.aux, nn, c->st));
-    HIPCHECK(hipMemsetAsync(c->f.T[k0], 0, sizeof(float) * nn, c->st));
-    c->prec = 1;
-    if ((rc = restart_accel<float>(c))) return rc;
-    const double target = std::max(kEta * rms, 0.5 * tol);
-    double r = 0.0, rbest = HUGE_VAL;
-    int stall = 0;
-    for (int inner = 0; istep < maxstep; ++inner) {
+    if ((rc = rebase_down(c, k0))) return rc;
+    if ((rc = accel_step<float>(c))) return rc;
+    for (int inner = 0; inner < block && istep < maxstep; ++inner) {
       ++istep;
+      double r = 0.0;
       if ((rc = calc_step<float>(c, &r, true))) return rc;
       ++low_steps;
       if (verbose)
         std::cout << " Step = " << istep << " Reside = " << r << " (FP32 correction)" << std::endl;
-      if (r < 0.99 * rbest) {
-        rbest = r;
-        stall = 0;
-      } else {
-        ++stall;
-      }
-      if (!std::isfinite(r) || r <= target || stall >= kStall || inner + 1 >= kMaxInner) break;
+      // a non-finite FP32 residual is not fed to the accelerator: the FP64
+      // step re-evaluates the same point
+      if (!std::isfinite(r)) break;
       if ((rc = accel_step<float>(c))) return rc;
+      if (r <= 0.5 * tol) break;
     }
-    // t0 += d; the next FP64 step evaluates the corrected t0
-    HIPCHECK(rism::run_add_correction(c->d.T[k0], c->f.T[c->iT_cur], nn, c->st));
-    c->iT_cur = c->iT_last = k0;
-    c->prec = c->last_prec = 0;
-    if ((rc = restart_accel<double>(c))) return rc;
+    if ((rc = rebase_up(c, k0))) return rc;
   }
   return RISM_OK;
 }
@@ -1142,6 +1171,10 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
     *err = rism::set_error(RISM_ERR_ARG, "anderson_m must be in [0, 16]");
     return nullptr;
   }
+  if (opts->mixed_block < 0) {
+    *err = rism::set_error(RISM_ERR_ARG, "mixed_block must be >= 0");
+    return nullptr;
+  }
   rism_ctx *c = new rism_ctx();
   auto fail = [&](int code) {
     *err = code;
@@ -1308,8 +1341,6 @@ int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, do
     if (rc) return rc;
     last = istep;
     if (fallback) {  // plain FP64 steps from the last FP64 evaluation on
-      if ((rc = restart_accel<double>(c))) return rc;
-      c->fresh = true;
       if ((rc = accel_step<double>(c))) return rc;
     }
   }

@@ -120,8 +120,13 @@ hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st);
 // closure exponent a = -u + t at the base point t0 of a correction sweep,
 // rounded to FP32
 hipError_t run_exponent(const double *t, const double *u, size_t n, float *out, hipStream_t st);
-// t0 += d (the FP32 correction of the defect-correction step)
-hipError_t run_add_correction(double *t, const float *d, size_t n, hipStream_t st);
+// fields of a correction block, whose FP32 values are differences from the
+// FP64 base point t0: up writes out = t0 + d (out may be t0), down writes
+// out = x - t0 rounded to FP32
+hipError_t run_rebase_up(double *out, const double *t0, const float *d, size_t n,
+                         hipStream_t st);
+hipError_t run_rebase_down(const double *x, const double *t0, float *out, size_t n,
+                           hipStream_t st);
 hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
                            double ubeta, int do_coul, double *u, double *fr, hipStream_t st);
 hipError_t run_fk_half(const double *atoms, int natu, int astride, const Params &P,

@@ -192,9 +192,18 @@ __global__ void k_exponent(const double *__restrict__ t, const double *__restric
   if (i < n) out[i] = (float)(-u[i] + t[i]);
 }
 
-__global__ void k_add_correction(double *__restrict__ t, const float *__restrict__ d, size_t n) {
+// Moves a field between the FP64 frame and the FP32 frame of a correction
+// block, whose values are differences from the base point t0:
+// up: out = t0 + d (out may be t0 itself); down: out = x - t0 in FP32.
+__global__ void k_rebase_up(double *out, const double *t0, const float *__restrict__ d,
+                            size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) t[i] += (double)d[i];
+  if (i < n) out[i] = t0[i] + (double)d[i];
+}
+__global__ void k_rebase_down(const double *__restrict__ x, const double *__restrict__ t0,
+                              float *__restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)(x[i] - t0[i]);
 }
 
 // conversion between the FP64 and FP32 copies of a field
@@ -680,8 +689,14 @@ hipError_t run_exponent(const double *t, const double *u, size_t n, float *out, 
   hipLaunchKernelGGL(k_exponent, dim3(nblk(n, 256)), dim3(256), 0, st, t, u, n, out);
   return hipGetLastError();
 }
-hipError_t run_add_correction(double *t, const float *d, size_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_add_correction, dim3(nblk(n, 256)), dim3(256), 0, st, t, d, n);
+hipError_t run_rebase_up(double *out, const double *t0, const float *d, size_t n,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_rebase_up, dim3(nblk(n, 256)), dim3(256), 0, st, out, t0, d, n);
+  return hipGetLastError();
+}
+hipError_t run_rebase_down(const double *x, const double *t0, float *out, size_t n,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_rebase_down, dim3(nblk(n, 256)), dim3(256), 0, st, x, t0, out, n);
   return hipGetLastError();
 }
 template hipError_t run_convert<float, double>(const float *, double *, size_t, hipStream_t);

@@ -79,7 +79,7 @@ class Opts(ctypes.Structure):
                 ("ad_switch", ctypes.c_int), ("lambda_", ctypes.c_double),
                 ("verbose", ctypes.c_int), ("chi_spline", ctypes.c_int),
                 ("anderson_m", ctypes.c_int), ("precision", ctypes.c_int),
-                ("mixed_switch", ctypes.c_double)]
+                ("mixed_switch", ctypes.c_double), ("mixed_block", ctypes.c_int)]
 
 
 class Thermo(ctypes.Structure):
@@ -314,7 +314,7 @@ class RISM3D:
 
     def __init__(self, grid, box, solvent, solute, closure=KH, count=2, m=0.6, mp=0.2,
                  ad_switch=0, lam=1.0, verbose=False, chi_spline=False, anderson_m=0,
-                 precision=0, mixed_switch=0.0):
+                 precision=0, mixed_switch=0.0, mixed_block=0):
         self.grid = [int(g) for g in grid]
         self.box = [float(b) for b in box]
         self.natv = int(solvent["natv"])
@@ -335,7 +335,7 @@ class RISM3D:
         g = Grid(self.grid[0], self.grid[1], self.grid[2], (ctypes.c_double * 3)(*self.box))
         op = Opts(int(closure), int(count), float(m), float(mp), int(ad_switch), float(lam),
                   int(bool(verbose)), int(bool(chi_spline)), int(anderson_m), int(precision),
-                  float(mixed_switch))
+                  float(mixed_switch), int(mixed_block))
         err = ctypes.c_int(0)
         self._ctx = lib().rism_create(ctypes.byref(g), ctypes.byref(sv), ctypes.byref(su),
                                       ctypes.byref(op), ctypes.byref(err))
@@ -346,10 +346,11 @@ class RISM3D:
     @classmethod
     def from_inp(cls, control, structure=None, center=True, q0=False, solvent_dir=None,
                  ad_switch=0, lam=1.0, verbose=False, chi_spline=False, anderson_m=0,
-                 precision=0, mixed_switch=0.0):
+                 precision=0, mixed_switch=0.0, mixed_block=0):
         d = read_input(control, structure, center, q0, solvent_dir)
         obj = cls(d["grid"], d["box"], d, d, d["closure"], d["count"], d["m"], d["mp"],
-                  ad_switch, lam, verbose, chi_spline, anderson_m, precision, mixed_switch)
+                  ad_switch, lam, verbose, chi_spline, anderson_m, precision, mixed_switch,
+                  mixed_block)
         obj.input = d
         obj.shift = d["shift"]
         return obj

@@ -107,9 +107,11 @@ typedef struct {
   int precision;         /* 0: FP64 throughout (the reference); 1: mixed --   */
                          /* FFTs, closure, OZ and the accelerator's fields in */
                          /* FP32 with FP64 sums, then FP64 steps to tol       */
-  double mixed_switch;   /* rms at which the mixed solve moves to FP64        */
-                         /* (<= tol: at tol); it also moves when the FP32 rms */
-                         /* stops improving                                   */
+  double mixed_switch;   /* rms at which the mixed solve leaves plain FP32    */
+                         /* steps for blocks of difference-form FP32 steps    */
+                         /* between FP64 steps (0: 1e-3); it also leaves when */
+                         /* the FP32 rms stops improving                      */
+  int mixed_block;       /* FP32 steps per block (0: 24)                      */
 } rism_opts;
 
 /* Thermodynamics of the converged state (output.cu:23-46, output_xmu.cu). */

@@ -71,15 +71,20 @@ def test_precision_transfer(rism, oracle, gpu, case24):
     h.close()
 
 
-@pytest.mark.parametrize("closure", ["KH", "HNC"])
-def test_mixed_solve_matches_fp64(rism, oracle, gpu, case24, closure):
+@pytest.mark.parametrize("closure,am,block", [("KH", 0, 0), ("HNC", 0, 0), ("KH", 5, 0),
+                                              ("KH", 0, 3)])
+def test_mixed_solve_matches_fp64(rism, oracle, gpu, case24, closure, am, block):
+    """Mixed solve (FP32 phase, then FP32 difference-form blocks between FP64
+    steps with the accelerator state carried across) against the FP64 solve:
+    AN2 and Anderson(5), the default block and a block of 3 FP32 steps."""
     sol, s, d, su, grid, box = case24
     q = 1.0 if closure == "KH" else 0.0  # HNC diverges with the full charges
     su = dict(su, uq=su["uq"] * q)
     clos = {"KH": rism.KH, "HNC": rism.HNC}[closure]
     out = {}
     for prec in (0, 1):
-        g = rism.RISM3D(grid, box, d, su, closure=clos, precision=prec)
+        g = rism.RISM3D(grid, box, d, su, closure=clos, precision=prec, anderson_m=am,
+                        mixed_block=block)
         th, tm = g.run(maxstep=2000, tol=1e-6)
         assert tm["status"] == rism.STATUS_CONVERGED, (prec, tm)
         assert g.precision() == 0
@@ -87,7 +92,7 @@ def test_mixed_solve_matches_fp64(rism, oracle, gpu, case24, closure):
         g.close()
     th64, tm64, h64 = out[0]
     th32, tm32, h32 = out[1]
-    print(closure, "FP64 steps", tm64["steps"], "mixed steps", tm32["steps"], "of them FP32",
+    print(closure, am, block, "FP64 steps", tm64["steps"], "mixed steps", tm32["steps"], "of them FP32",
           tm32["steps_fp32"], "dSFE", th32["SFE_SC"] - th64["SFE_SC"])
     assert tm32["steps_fp32"] > tm32["steps"] // 2
     assert abs(th32["SFE_SC"] - th64["SFE_SC"]) < 0.01 * KCAL
