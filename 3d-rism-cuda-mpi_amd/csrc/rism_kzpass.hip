@@ -89,8 +89,22 @@ struct ZinvSums {
   static constexpr int W = HM == 0 ? 6 : 1 + RISM_MAX_HIST;
 };
 
+// Waves per SIMD the residual pass is compiled for.  An FP32 tile (<= 74 KB
+// of LDS, 1024 threads) leaves room for a second workgroup per CU once the
+// kernel fits in 64 VGPRs, and the FP64 Anderson(m) tile (512 threads, 74 KB)
+// once it fits in 128; with one workgroup per CU the load, transform and
+// residual phases of a tile do not overlap.  The compiler spills 36-116 B
+// per lane to meet these limits, far cheaper than the lost overlap (512^3,
+// 4 sites: FP32 4.7 -> 3.2 ms, FP64 Anderson(10) 24.4 -> 14.9 ms).  The FP64
+// AN2 tile needs 148 KB of LDS and stays at one workgroup per CU.
+template <int HM, class Re>
+constexpr int zinv_waves_per_eu() {
+  return sizeof(Re) == 4 ? 8 : (HM ? 4 : 1);
+}
+
 template <int M, int B, int TZ, int HM, class Re>
 __global__ void __launch_bounds__(B *TZ)
+    __attribute__((amdgpu_waves_per_eu(zinv_waves_per_eu<HM, Re>())))
     k_zinv_resid(const cx<Re> *__restrict__ spec, const Re *__restrict__ t,
                  const Re *__restrict__ u, Re *__restrict__ R, const Re *__restrict__ R0,
                  const Re *__restrict__ R1, const Re *__restrict__ r0, HistT<Re> H, Params P,
