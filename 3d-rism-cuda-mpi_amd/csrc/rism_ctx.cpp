@@ -1519,26 +1519,37 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
   if (!ctx || n < 0) return rism::set_error(RISM_ERR_ARG, "null argument");
   for (int i = 0; i < n; ++i) {
     if (!ctx[i]) return rism::set_error(RISM_ERR_ARG, "null context in batch");
-    if (!async_capable(ctx[i]) || ctx[i]->prec != 0)
-      return rism::set_error(RISM_ERR_ARG,
-                             "batched solves run FP64 AN2 contexts without verbose output");
+    for (int j = 0; j < i; ++j)
+      if (ctx[j] == ctx[i]) return rism::set_error(RISM_ERR_ARG, "context repeated in batch");
   }
   if (maxstep <= 0) return rism::set_error(RISM_ERR_ARG, "maxstep must be positive");
   if (n == 0) return RISM_OK;
   std::vector<AsyncRun> runs(n);
   const int nt = std::max(1, std::min(threads > 0 ? threads : 1, n));
   std::vector<int> rcs(nt, RISM_OK);
-  // worker w drives contexts w, w + nt, ...: enqueue a chunk on each of its
-  // streams, then drain them in turn, until every run has finished
+  // worker w drives contexts w, w + nt, ...  FP64 AN2 contexts run without a
+  // host round trip per step: enqueue a chunk on each of their streams, then
+  // drain them in turn, until every run has finished.  Contexts whose
+  // accelerator solves on the host (mixed precision, Anderson(m)) then run
+  // rism_solve's step-by-step loop on their own streams, concurrently with
+  // the other workers' contexts.
   auto work = [&](int w) {
     int rc = RISM_OK;
-    for (int i = w; i < n && !rc; i += nt) rc = async_begin(runs[i], ctx[i], maxstep, tol);
+    std::vector<int> fast, slow;
+    for (int i = w; i < n; i += nt) (async_capable(ctx[i]) ? fast : slow).push_back(i);
+    for (int i : fast) {
+      if (rc) break;
+      rc = async_begin(runs[i], ctx[i], maxstep, tol);
+    }
     bool left = true;
     while (!rc && left) {
       left = false;
-      for (int i = w; i < n && !rc; i += nt)
+      for (int i : fast) {
+        if (rc) break;
         if (!runs[i].finished) rc = async_enqueue(runs[i], kAsyncChunk);
-      for (int i = w; i < n && !rc; i += nt) {
+      }
+      for (int i : fast) {
+        if (rc) break;
         if (runs[i].finished) continue;
         if (hipStreamSynchronize(ctx[i]->st) != hipSuccess)
           rc = rism::set_error(RISM_ERR_HIP, "hipStreamSynchronize failed");
@@ -1546,7 +1557,11 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
         left = left || !runs[i].finished;
       }
     }
-    for (int i = w; i < n; i += nt) ctx[i]->async = false;
+    for (int i : fast) ctx[i]->async = false;
+    for (int i : slow) {
+      if (rc) break;
+      rc = rism_solve(ctx[i], maxstep, tol, &runs[i].status, &runs[i].steps, &runs[i].rms);
+    }
     rcs[w] = rc;
   };
   std::vector<std::thread> pool;

@@ -155,3 +155,22 @@ def batch_grid(xyz, sizes=BATCH_SIZES, spacing=BATCH_SPACING, pad=BATCH_PAD):
         fit = [n for n in sizes if n * spacing >= e + 2 * pad]
         grid.append(fit[0] if fit else sizes[-1])
     return tuple(grid), tuple(n * spacing for n in grid)
+
+
+def cfg3_solute():
+    """3000-atom protein-like cluster, R = 19 A, net charge 0."""
+    return cluster_solute(3000, 19.0, seed=1003, total_charge=0.0)
+
+
+# per-axis grid sizes of the batched medium-size workload
+MEDIUM_SIZES = (128, 144, 160, 180, 192)
+
+
+def medium_solute(i, seed0=1005000):
+    """Solute i of the batched medium-size workload: N log-uniform on
+    [500, 4000], a cluster of radius (3N / (4 pi 0.1))^(1/3) (0.1 atoms per
+    A^3), net charge 0, from seed seed0 + i."""
+    rng = np.random.default_rng(seed0 + i)
+    n = int(round(np.exp(rng.uniform(np.log(500.0), np.log(4000.0)))))
+    radius = (3.0 * n / (4.0 * np.pi * 0.1)) ** (1.0 / 3.0)
+    return cluster_solute(n, radius, seed=seed0 + i, total_charge=0.0)

@@ -17,7 +17,8 @@ the timed solves), and an FP64 CPU baseline (the oracle, a plain-C OpenMP
 restatement of the reference algorithm) timed on a bounded sample on the
 host cores and extrapolated to one solve.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2|cfg1|mobley]
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+                  [--workload cfg2|cfg1|cfg3|mobley|batch|batch-medium]
 """
 import argparse
 import json
@@ -58,6 +59,13 @@ def workload(name):
         desc = ("cfg1: 64^3 / 32 A, 24-atom random-walk solute (seed 1001), TIP3P water "
                 "(chi from the project's 1D DRISM-KH, eps 78.4, 298.15 K), KH, AN2 2 0.6 0.2, "
                 "tol 1e-6, FP64")
+    elif name == "cfg3":
+        sol = G.cfg3_solute()
+        xvk = "tip3p_kcl01_drism.xvk"
+        grid, box = (256, 256, 256), (128.0, 128.0, 128.0)
+        desc = ("cfg3: 256^3 / 128 A, synthetic 3000-atom cluster (R 19 A, 0 e, seed 1003), "
+                "0.1 M KCl in TIP3P (chi from the project's 1D DRISM-KH, eps 78.4, 298.15 K, "
+                "4 sites), tol 1e-6, FP64")
     elif name == "mobley":
         d = rism.read_input(os.path.join(GOLDEN, "3drism.dat"),
                             os.path.join(GOLDEN, "mobley_3053621"), solvent_dir=GOLDEN)
@@ -160,6 +168,114 @@ def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv, budget_s=30.0):
     }
 
 
+def closure_code(rism, args, default):
+    name = args.closure or default
+    return {"KH": rism.KH, "HNC": rism.HNC, "PSE3": rism.PSE(3)}[name]
+
+
+def bench_medium(args):
+    """Batched medium-size solutes in mixed precision: 32 protein-like
+    clusters (500-4000 atoms log-uniform, 0.1 atoms/A^3, net charge 0, seeds
+    1005000 + i) in 0.2 M NaCl TIP3P (tip3p_nacl02_drism.xvk), boxes of
+    extent + 12 A per side at 0.5 A spacing with per-axis sizes from
+    {128, 144, 160, 180, 192}; HNC, AN2 2 0.6 0.2, tol 1e-6.  Times the
+    mixed-precision batch (rism_run_batch) against the same mixed solves one
+    after another, and reports |dSFE| of every solute against its FP64 solve
+    (run as a sync-free FP64 batch, also timed)."""
+    import torch
+    import gen_inputs as G
+    import __graft_entry__ as ge
+    rism = ge.load_rism3d()
+    solv = rism.read_xvk(os.path.join(GOLDEN, "tip3p_nacl02_drism.xvk"))
+    nsol = args.batch_size if args.batch_size else 32
+    clos = closure_code(rism, args, "HNC")
+    cname = args.closure or "HNC"
+    cases, natoms = [], []
+    for i in range(nsol):
+        sol = G.medium_solute(i)
+        grid, box = G.batch_grid(sol["xyz"], sizes=G.MEDIUM_SIZES)
+        xyz = sol["xyz"]
+        mn, mx = xyz.min(0), xyz.max(0)
+        shift = np.array([np.floor(-(mx[k] - mn[k]) / 2 - mn[k] + 0.5) for k in range(3)])
+        cases.append((grid, box, {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"],
+                                  "uxyz": xyz + shift}))
+        natoms.append(len(sol["sig"]))
+    log(f"[bench] batch-medium: {nsol} solutes ({min(natoms)}-{max(natoms)} atoms), "
+        f"{len(set(c[0] for c in cases))} grid shapes")
+
+    def make(prec):
+        return [rism.RISM3D(g, b, solv, su, clos, 2, 0.6, 0.2, precision=prec)
+                for g, b, su in cases]
+
+    def timed_batch(prec):
+        gs = make(prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = rism.run_batch(gs, threads=args.batch_threads)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        for g in gs:
+            g.close()
+        return t, out
+
+    w = make(1)[:2]  # warm-up (module load, first launches)
+    rism.run_batch(w, threads=args.batch_threads)
+    for g in w:
+        g.close()
+    t_mixed, out_m = timed_batch(1)
+    log(f"[bench] mixed batch: {t_mixed:.2f} s")
+    gs = make(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq = [g.run()[0]["SFE_SC"] for g in gs]
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    for g in gs:
+        g.close()
+    log(f"[bench] mixed one after another: {t_seq:.2f} s")
+    t_fp64, out_d = timed_batch(0)
+    log(f"[bench] FP64 batch: {t_fp64:.2f} s")
+    conv = [i for i in range(nsol) if out_m[i][1] == rism.STATUS_CONVERGED
+            and out_d[i][1] == rism.STATUS_CONVERGED]
+    dmu = np.array([abs(out_m[i][0]["SFE_SC"] - out_d[i][0]["SFE_SC"]) / 4184.0 for i in conv])
+    steps = np.array([o[2] for o in out_m])
+    res = {
+        "metric": METRIC,
+        "value": nsol / t_mixed,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1000.0 * t_mixed,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bulk, f64 sums and FP64 steps",
+        "data": "synthetic",
+        "config": {
+            "workload": ("batch-medium: %d protein-like clusters (500-4000 atoms, seeds "
+                         "1005000+i, 0 e) in 0.2 M NaCl TIP3P (chi from the project's 1D "
+                         "DRISM-KH, eps 78.4), boxes = extent + 12 A/side at 0.5 A, grids "
+                         "from {128..192}^3, %s, AN2 2 0.6 0.2, tol 1e-6, mixed precision"
+                         % (nsol, cname)),
+            "solves": nsol, "grid_shapes": len(set(c[0] for c in cases)),
+            "atoms_min_max": [min(natoms), max(natoms)],
+            "batch_s": t_mixed, "sequential_s": t_seq,
+            "speedup_vs_sequential": t_seq / t_mixed,
+            "batch_equals_sequential": bool(all(out_m[i][0]["SFE_SC"] == seq[i]
+                                                for i in range(nsol))),
+            "fp64_batch_s": t_fp64, "speedup_vs_fp64_batch": t_fp64 / t_mixed,
+            "iterations_p50": float(np.percentile(steps, 50)),
+            "iterations_p99": float(np.percentile(steps, 99)),
+            "converged_mixed": int(sum(1 for o in out_m if o[1] == rism.STATUS_CONVERGED)),
+            "converged_fp64": int(sum(1 for o in out_d if o[1] == rism.STATUS_CONVERGED)),
+            "dSFE_kcal_per_mol_max": float(dmu.max()) if len(dmu) else None,
+            "dSFE_kcal_per_mol_p50": float(np.percentile(dmu, 50)) if len(dmu) else None,
+            "within_0.05_kcal_per_mol": int((dmu <= 0.05).sum()),
+            "host_threads": args.batch_threads,
+        },
+    }
+    print(json.dumps(res), flush=True)
+
+
 def bench_batch(args):
     """Batched small solutes: 256 random-walk solutes of 8-60 atoms (seeds
     1004000 + i, net charge -1/0/+1) in TIP3P water (tip3p_drism.xvk), each in
@@ -172,7 +288,7 @@ def bench_batch(args):
     import __graft_entry__ as ge
     rism = ge.load_rism3d()
     solv = rism.read_xvk(os.path.join(GOLDEN, "tip3p_drism.xvk"))
-    nsol = args.batch_size
+    nsol = args.batch_size if args.batch_size else 256
     cases = []
     for i in range(nsol):
         sol = G.batch_solute(i)
@@ -268,8 +384,15 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg2",
-                    help="cfg2 (512^3, default), cfg1, mobley, or batch (256 small solutes)")
-    ap.add_argument("--batch-size", type=int, default=256)
+                    choices=["cfg2", "cfg1", "cfg3", "mobley", "batch", "batch-medium"],
+                    help="cfg2 (512^3, default), cfg1 (64^3), cfg3 (256^3, PSE-3), mobley, "
+                         "batch (256 small solutes) or batch-medium (32 medium solutes, "
+                         "mixed precision)")
+    ap.add_argument("--closure", default=None, choices=["KH", "HNC", "PSE3"],
+                    help="override the workload's closure (cfg3: PSE3, batch-medium: HNC, "
+                         "others: KH)")
+    ap.add_argument("--batch-size", type=int, default=None,
+                    help="solutes of a batch workload (256 small / 32 medium)")
     ap.add_argument("--batch-threads", type=int, default=8)
     ap.add_argument("--anderson-m", type=int, default=None,
                     help="history length of Anderson(m) mixing (0: the reference's AN2)")
@@ -306,12 +429,16 @@ def main():
     if args.workload == "batch":
         bench_batch(args)
         return
+    if args.workload == "batch-medium":
+        bench_medium(args)
+        return
     rism, grid, box, solv, sol, xvk, desc = workload(args.workload)
     solu = {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"], "uxyz": sol["xyz"]}
     log(f"[bench] workload {args.workload}: grid {grid}, natu {len(sol['sig'])}, "
         f"natv {solv['natv']}")
     am = args.anderson_m if args.anderson_m is not None else 0
-    g = rism.RISM3D(grid, box, solv, solu, rism.KH, 2, args.beta, 0.2, anderson_m=am)
+    clos = closure_code(rism, args, "PSE3" if args.workload == "cfg3" else "KH")
+    g = rism.RISM3D(grid, box, solv, solu, clos, 2, args.beta, 0.2, anderson_m=am)
     for i in range(args.warmup):
         th, tm = g.run(maxstep=args.maxstep, tol=1e-6)
         log(f"[bench] warmup {i}: {tm}")
@@ -327,6 +454,9 @@ def main():
     wall = time.perf_counter() - t0
     ks = g.kernel_stats()
     g.kernel_timing(False)
+    N = grid[0] * grid[1] * grid[2]
+    # h of the FP64 solve for the rms(dh) of the mixed one (grids up to 256^3)
+    h64 = g.fields(rism.FIELD_H) if N <= 256 ** 3 else None
     g.close()
     th, tm = results[-1]
     if tm["status"] != rism.STATUS_CONVERGED:
@@ -352,7 +482,7 @@ def main():
 
     mixed = None
     if args.precision in ("mixed", "both"):
-        gm = rism.RISM3D(grid, box, solv, solu, rism.KH, 2, args.beta, 0.2, anderson_m=am,
+        gm = rism.RISM3D(grid, box, solv, solu, clos, 2, args.beta, 0.2, anderson_m=am,
                          precision=1)
         for i in range(args.warmup):
             thm, tmm = gm.run(maxstep=args.maxstep, tol=1e-6)
@@ -367,6 +497,8 @@ def main():
         wall_m = time.perf_counter() - t1
         ksm = gm.kernel_stats()
         gm.kernel_timing(False)
+        rms_dh = (float(np.sqrt(np.mean((gm.fields(rism.FIELD_H) - h64) ** 2)))
+                  if h64 is not None else None)
         gm.close()
         dmu = thm["SFE_SC"] - th["SFE_SC"]
         mixed = {
@@ -379,6 +511,7 @@ def main():
             "dSFE_SC_kcal_per_mol": dmu / 4184.0,
             "gate_0.05_kcal_per_mol": abs(dmu) <= 0.05 * 4184.0,
             "dPMV": thm["PMV"] - th["PMV"],
+            "rms_dh": rms_dh,
             "kernels_avg_ms": {k: v[1] / max(v[0], 1) for k, v in ksm.items() if v[0] > 0},
         }
         log(f"[bench] mixed: {mixed}")
@@ -405,7 +538,9 @@ def main():
         "config": {
             "workload": desc,
             "grid": list(grid), "box": list(box), "natu": int(len(sol["sig"])),
-            "natv": int(natv), "anderson_m": am, "iterations": iters, "status": tm["status"],
+            "natv": int(natv), "closure": args.closure or ("PSE3" if args.workload == "cfg3"
+                                                            else "KH"),
+            "anderson_m": am, "iterations": iters, "status": tm["status"],
             "setup_ms": tm["setup_ms"], "iterate_ms": tm["iterate_ms"],
             "thermo_ms": tm["thermo_ms"], "ms_per_iteration": iter_ms,
             "iteration_GBps_algorithmic": iter_bytes / (iter_ms / 1000.0) / 1e9,

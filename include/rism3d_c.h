@@ -173,12 +173,14 @@ int rism_calculate(rism_ctx *ctx, double *rms);
 int rism_anderson(rism_ctx *ctx);
 /* ---- batches of independent solves (no counterpart in the reference,
  * which runs one solute per process) ------------------------------------ */
-/* Runs rism_solve on n contexts at once: each context's steps are enqueued
- * on its own stream in chunks, without a host round trip per step (a device
- * flag stops a context's kernels at convergence), from `threads` host
- * threads.  The results are those of n separate rism_solve calls.  The
- * contexts must be FP64 (precision 0) with AN2 (anderson_m 0) and verbose
- * 0.  status/steps/rms: n entries each (may be NULL). */
+/* Runs rism_solve on n distinct contexts at once from `threads` host
+ * threads, each context on its own stream.  FP64 AN2 contexts (precision 0,
+ * anderson_m 0, verbose 0) have their steps enqueued in chunks without a
+ * host round trip per step (a device flag stops a context's kernels at
+ * convergence); the others (mixed precision, Anderson(m), verbose) run
+ * rism_solve's step-by-step loop on their worker thread.  The results are
+ * those of n separate rism_solve calls.  status/steps/rms: n entries each
+ * (may be NULL). */
 int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads,
                      int *status, int *steps, double *rms);
 /* Potentials, batched solve and thermodynamics for n contexts (rism_run for

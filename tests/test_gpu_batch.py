@@ -87,12 +87,37 @@ def test_batch_mixed_outcomes(rism, gpu, golden):
         h.close()
 
 
-def test_batch_rejects_unsupported(rism, gpu, golden):
+def test_batch_with_host_accelerated_contexts(rism, gpu, golden):
+    """Mixed-precision and Anderson(m) contexts, whose accelerators solve on
+    the host every step, run in a batch beside FP64 AN2 contexts (the
+    step-by-step loop on their worker thread): every member equals its
+    separate solve bit for bit."""
+    solv = _solvent(rism, golden)
+    cases = [_case(i) for i in range(4)]
+    kinds = [dict(), dict(precision=1), dict(anderson_m=4), dict(precision=1, anderson_m=3)]
+    ref = []
+    for (g, b, su), kw in zip(cases, kinds):
+        h = rism.RISM3D(g, b, solv, su, **kw)
+        th, tm = h.run(maxstep=1000, tol=1e-6)
+        ref.append((th, tm, h.fields(rism.FIELD_T)))
+        h.close()
+    hs = [rism.RISM3D(g, b, solv, su, **kw) for (g, b, su), kw in zip(cases, kinds)]
+    out = rism.run_batch(hs, maxstep=1000, tol=1e-6, threads=2)
+    for (th, st, steps), h, (thr, tmr, tr), kw in zip(out, hs, ref, kinds):
+        assert st == tmr["status"] == rism.STATUS_CONVERGED, kw
+        assert steps == tmr["steps"], kw
+        for k in ("SFE_SC", "SFE_GF", "PMV"):
+            assert th[k] == thr[k], (kw, k)
+        assert np.array_equal(h.fields(rism.FIELD_T), tr), kw
+        h.close()
+
+
+def test_batch_rejects_repeated_context(rism, gpu, golden):
     solv = _solvent(rism, golden)
     g, b, su = _case(1)
-    h = rism.RISM3D(g, b, solv, su, precision=1)
+    h = rism.RISM3D(g, b, solv, su)
     h.initialize()
     with pytest.raises(rism.RismError) as e:
-        rism.solve_batch([h])
+        rism.solve_batch([h, h])
     assert e.value.code == rism.RISM_ERR_ARG
     h.close()
