@@ -235,3 +235,64 @@ def test_analysis_outputs(rism, oracle, gpu, case24, ad, lam):
         assert got.shape == ref.shape
         assert np.abs(got - ref).max() < 1e-11 * scale, (kind, np.abs(got - ref).max(), scale)
     g.close()
+
+
+def _steps(g, n):
+    """n reference loop steps (iterate.cu:48-72): calculate, then the AN2 update."""
+    out = []
+    for _ in range(n):
+        out.append(g.calculate())
+        g.anderson()
+    return out
+
+
+def test_tuv_checkpoint_restart(rism, oracle, gpu, case24, tmp_path):
+    """.tuv checkpoint (write_tuv.cu:19-28, read_tuv.cu:6-29, restored t
+    uploaded): the file holds the header and t at the stream's default 6
+    significant digits; a context restarted from it runs the same residual
+    sequence bit for bit as one given those values directly, and both reach
+    the fixed point of an uninterrupted solve."""
+    sol, s, d, grid, box = case24
+    su = _solute(sol)
+    g = rism.RISM3D(grid, box, d, su)
+    g.initialize()
+    _steps(g, 30)
+    path = tmp_path / "case.tuv"
+    g.write_tuv(path)
+    t_mem = g.fields(rism.FIELD_T)
+    g.close()
+    tok = path.read_text().split()
+    assert [int(x) for x in tok[:4]] == [grid[0], grid[1], grid[2], d["natv"]]
+    saved = np.array(tok[4:], dtype=float).reshape(d["natv"], -1)
+    six = np.array([float(f"{x:g}") for x in t_mem.ravel()]).reshape(saved.shape)
+    assert np.array_equal(saved, six)
+    # restart from the file against the same values uploaded directly
+    a = rism.RISM3D(grid, box, d, su)
+    a.initialize()
+    a.read_tuv(path)
+    assert np.array_equal(a.fields(rism.FIELD_T), saved)
+    b = rism.RISM3D(grid, box, d, su)
+    b.initialize()
+    for v in range(d["natv"]):
+        b.set(rism.FIELD_T, v, saved[v])
+    ra, rb = _steps(a, 12), _steps(b, 12)
+    assert ra == rb
+    b.close()
+    st, n, rms = a.iterate(maxstep=2000, tol=1e-6)
+    assert st == rism.STATUS_CONVERGED
+    th_a = a.thermo()
+    a.close()
+    ref = rism.RISM3D(grid, box, d, su)
+    th_r, tm_r = ref.run(maxstep=2000, tol=1e-6)
+    ref.close()
+    assert tm_r["status"] == rism.STATUS_CONVERGED
+    # the same fixed point, to the tolerance band of a converged solve
+    assert abs(th_a["SFE_SC"] - th_r["SFE_SC"]) < 2.0
+    assert abs(th_a["PMV"] - th_r["PMV"]) < 1e-5
+    # a mismatched grid is refused (read_tuv.cu:14-20)
+    c = rism.RISM3D((48, 48, 40), (24.0, 24.0, 20.0), d, su)
+    c.initialize()
+    with pytest.raises(rism.RismError) as e:
+        c.read_tuv(path)
+    assert e.value.code == rism.RISM_ERR_ARG
+    c.close()
