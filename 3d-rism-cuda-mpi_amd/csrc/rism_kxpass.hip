@@ -176,8 +176,17 @@ __global__ void __launch_bounds__(512)
   const size_t rowoff = ((size_t)m * P.ny + y) * L;
   C *row = spec + (size_t)v * rowstride + rowoff;
   C *mylds = lds + v * CS;
-  auto load = [&](int i) -> C { return cmul(row[i], prex[i]); };
-  auto store = [&](int i, C vv) { row[i] = cmulc(vv, prex[i]); };
+  // the spectrum row is read once and written once: non-temporal accesses
+  // leave the caches to the chi table, fk and the twiddles
+  auto load = [&](int i) -> C {
+    const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
+    return cmul(a, prex[i]);
+  };
+  auto store = [&](int i, C vv) {
+    const C o = cmulc(vv, prex[i]);
+    __builtin_nontemporal_store(o.x, &row[i].x);
+    __builtin_nontemporal_store(o.y, &row[i].y);
+  };
   auto nostore = [&](int, C) {};
   const int nthr = nv * T;
   const int nv2 = nv * nv;

@@ -21,13 +21,20 @@ __global__ void __launch_bounds__(B *FftShape<L>::T)
   const int x = blockIdx.x * B + c;
   const bool ok = x < nx;
   C *col = spec + (size_t)blockIdx.y * L * nx + (ok ? x : 0);
+  // the spectrum is read once and written once per pass: non-temporal
+  // accesses leave the caches to the twiddle tables
   auto load = [&](int i) -> C {
     if (!ok) return C{0, 0};
-    const C vv = col[(size_t)i * nx];
+    const C *q = col + (size_t)i * nx;
+    const C vv{__builtin_nontemporal_load(&q->x), __builtin_nontemporal_load(&q->y)};
     return (S < 0) ? cmul(vv, pre[i]) : vv;
   };
   auto store = [&](int i, C vv) {
-    if (ok) col[(size_t)i * nx] = (S < 0) ? vv : cmulc(vv, pre[i]);
+    if (!ok) return;
+    const C o = (S < 0) ? vv : cmulc(vv, pre[i]);
+    C *q = col + (size_t)i * nx;
+    __builtin_nontemporal_store(o.x, &q->x);
+    __builtin_nontemporal_store(o.y, &q->y);
   };
   fft_column<L, S, true, true>(lds + c * CS, j, twL, load, store);
 }
