@@ -463,7 +463,6 @@ def main():
         log(f"[bench] WARNING: solve did not converge (status {tm['status']})")
     ms_per_step = 1000.0 * wall / args.steps
     natv = solv["natv"]
-    N = grid[0] * grid[1] * grid[2]
     iters = tm["steps"]
 
     # roofline of the dominant hot-loop kernel
@@ -478,6 +477,14 @@ def main():
                   for k, v in ks.items() if v[0] > 0}
     iter_bytes = sum(ab.values())
     iter_ms = tm["iterate_ms"] / max(iters, 1)
+    # the fused lower bound of one iteration, each 3D transform counted as
+    # one HBM round trip: N [(12 + 3 m) n + 2] w bytes, m history pairs (2
+    # for AN2), w bytes per bulk value
+    mh = am if am > 0 else 2
+
+    def fused_bytes(w):
+        return N * ((12 + 3 * mh) * natv + 2) * w
+
     log("[bench] per-kernel:", json.dumps(per_kernel, indent=None))
 
     mixed = None
@@ -512,6 +519,10 @@ def main():
             "gate_0.05_kcal_per_mol": abs(dmu) <= 0.05 * 4184.0,
             "dPMV": thm["PMV"] - th["PMV"],
             "rms_dh": rms_dh,
+            # FP32 steps at w = 4, FP64 steps at w = 8, over the iteration phase
+            "iteration_fused_bound_frac": ((tmm["steps_fp32"] * fused_bytes(4)
+                                            + (tmm["steps"] - tmm["steps_fp32"]) * fused_bytes(8))
+                                           / (tmm["iterate_ms"] / 1000.0) / (HBM_PEAK_GBS * 1e9)),
             "kernels_avg_ms": {k: v[1] / max(v[0], 1) for k, v in ksm.items() if v[0] > 0},
         }
         log(f"[bench] mixed: {mixed}")
@@ -544,6 +555,8 @@ def main():
             "setup_ms": tm["setup_ms"], "iterate_ms": tm["iterate_ms"],
             "thermo_ms": tm["thermo_ms"], "ms_per_iteration": iter_ms,
             "iteration_GBps_algorithmic": iter_bytes / (iter_ms / 1000.0) / 1e9,
+            "iteration_fused_bound_GB": fused_bytes(8) / 1e9,
+            "iteration_fused_bound_frac": fused_bytes(8) / (iter_ms / 1000.0) / (HBM_PEAK_GBS * 1e9),
             "SFE_SC": th["SFE_SC"], "SFE_GF": th["SFE_GF"], "PMV": th["PMV"],
             "kernels": per_kernel,
         },
