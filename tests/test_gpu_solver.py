@@ -296,3 +296,48 @@ def test_tuv_checkpoint_restart(rism, oracle, gpu, case24, tmp_path):
         c.read_tuv(path)
     assert e.value.code == rism.RISM_ERR_ARG
     c.close()
+
+
+def test_six_sites_one_step(rism, oracle, gpu, case24, golden):
+    """A 6-site solvent (water and the 4-site NaCl table side by side, chi
+    block-diagonal: a synthetic mixture that exercises the x pass's 8-site
+    instantiation, shell table and spline) against the oracle, one step."""
+    sol, s2, _, _, _ = case24
+    s4 = oracle.read_xvk(f"{golden}/naclaq_vv.xvk")
+    n = min(s2.ntab, s4.ntab)
+    assert s2.dr == s4.dr
+    x = np.zeros((n, 6, 6))
+    x[:, :2, :2] = s2.xvv[:n]
+    x[:, 2:, 2:] = s4.xvv[:n]
+    cat = lambda a, b: np.concatenate([a, b])
+    s6 = oracle.Solvent(6, n, s2.dr, s2.temper, s2.xt, cat(s2.sig, s4.sig), cat(s2.eps, s4.eps),
+                        cat(s2.q, s4.q), cat(s2.rho, s4.rho), x)
+    d6 = {"natv": 6, "ntab": n, "dr": s6.dr, "temper": s6.temper, "xt": s6.xt, "vsig": s6.sig,
+          "veps": s6.eps, "vq": s6.q, "vrho": s6.rho, "xvv": s6.xvv}
+    grid, box = (32, 32, 32), (16.0, 16.0, 16.0)
+    o = oracle.OracleRISM(grid, box, s6, sol, "KH").setup()
+    rms_o, g_o, R_o, h_o = o.calculate(o.t0)
+    for spline in (False, True):
+        g = rism.RISM3D(grid, box, d6, _solute(sol), chi_spline=spline)
+        g.initialize()
+        rms_g = g.calculate()
+        assert abs(rms_g - rms_o) < 1e-11 * rms_o, spline
+        for which, ref in ((rism.FIELD_R, R_o), (rism.FIELD_H, h_o)):
+            got = g.fields(which).reshape(-1)
+            assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max(), (which, spline)
+        g.close()
+
+
+def test_empty_solute(rism, oracle, gpu, case24):
+    """No solute atoms: u = fr = fk = 0 and t = 0, so g = 1, h = 0 and the
+    first residual is exactly zero; the solve stops at step 1 with a zero
+    excess chemical potential."""
+    _, _, d, grid, box = case24
+    empty = {"usig": np.zeros(0), "ueps": np.zeros(0), "uq": np.zeros(0),
+             "uxyz": np.zeros((0, 3))}
+    g = rism.RISM3D(grid, box, d, empty)
+    th, tm = g.run(maxstep=100, tol=1e-6)
+    assert (tm["status"], tm["steps"], tm["rms"]) == (rism.STATUS_CONVERGED, 1, 0.0)
+    assert not np.any(g.fields(rism.FIELD_H))
+    assert th["SFE_SC"] == 0.0 and th["SFE_GF"] == 0.0
+    g.close()
