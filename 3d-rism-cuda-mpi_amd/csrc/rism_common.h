@@ -106,6 +106,15 @@ struct ZTile {
   // the residual pass with the Anderson(m) sums needs more registers: half
   // as many columns (512 threads)
   static constexpr int BH = (M >= 128 && B > 1) ? B / 2 : B;
+  // FP64 forward pass of long columns in radix-4 stages: M/4 threads per
+  // column and half as many columns (74 KB of LDS), so that two 1024-thread
+  // workgroups share a CU (53 VGPRs, no spills; 4.8 -> 4.3 ms at 512^3, 4
+  // sites).  The residual pass gains nothing from the same plan (it would
+  // spill to fit 64 VGPRs) and keeps radix 8.
+  static constexpr bool R4F = sizeof(Re) == 8 && M >= 128;
+  static constexpr int MAXRF = R4F ? 4 : 8;
+  static constexpr int TF = R4F ? FftShape<M, 4>::T : T;
+  static constexpr int BF = R4F ? 1024 / TF : B;
 };
 
 static inline unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }

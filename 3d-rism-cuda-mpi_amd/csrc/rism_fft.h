@@ -160,20 +160,28 @@ struct RadixList {
   int r[16];
 };
 
-constexpr RadixList make_radices(int L) {
+// maxr = 8: as many radix-8 stages as possible; maxr = 4: radix-4 stages
+// (twice the threads per column, smaller butterflies)
+constexpr RadixList make_radices(int L, int maxr = 8) {
   RadixList rl{0, {}};
   int m = L;
   int n8 = 0, n4 = 0, n2 = 0;
   while (m % 2 == 0) { ++n2; m /= 2; }
-  // powers of two: as many radix-8 as possible, remainder as 4 / 4+? / 2
-  n8 = n2 / 3;
-  int rem = n2 % 3;
-  if (rem == 1) {
-    if (n8 > 0) { n8 -= 1; n4 = 2; } else { n2 = 1; }
-  } else if (rem == 2) {
-    n4 = 1;
+  int twos = 0;
+  if (maxr == 4) {
+    n4 = n2 / 2;
+    twos = n2 % 2;
+  } else {
+    // powers of two: as many radix-8 as possible, remainder as 4 / 4+? / 2
+    n8 = n2 / 3;
+    int rem = n2 % 3;
+    if (rem == 1) {
+      if (n8 > 0) { n8 -= 1; n4 = 2; } else { n2 = 1; }
+    } else if (rem == 2) {
+      n4 = 1;
+    }
+    twos = (rem == 1 && n4 == 0) ? 1 : 0;
   }
-  int twos = (rem == 1 && n4 == 0) ? 1 : 0;
   for (int i = 0; i < n8; ++i) rl.r[rl.n++] = 8;
   for (int i = 0; i < n4; ++i) rl.r[rl.n++] = 4;
   for (int i = 0; i < twos; ++i) rl.r[rl.n++] = 2;
@@ -183,8 +191,8 @@ constexpr RadixList make_radices(int L) {
   return rl;
 }
 
-constexpr int max_radix(int L) {
-  RadixList rl = make_radices(L);
+constexpr int max_radix(int L, int maxr = 8) {
+  RadixList rl = make_radices(L, maxr);
   int mx = 1;
   for (int i = 0; i < rl.n; ++i) mx = rl.r[i] > mx ? rl.r[i] : mx;
   return mx;
@@ -192,11 +200,11 @@ constexpr int max_radix(int L) {
 
 // threads per column: enough for one radix-max butterfly each in the
 // largest-radix stages
-template <int L>
+template <int L, int MAXR = 8>
 struct FftShape {
-  static constexpr RadixList rl = make_radices(L);
+  static constexpr RadixList rl = make_radices(L, MAXR);
   static_assert(rl.n > 0, "FFT length must be 2^a 3^b 5^c");
-  static constexpr int T = L / max_radix(L);
+  static constexpr int T = L / max_radix(L, MAXR);
   static constexpr int CS = L + L / 8 + 1;  // padded LDS column stride
 };
 
@@ -254,18 +262,18 @@ __device__ __forceinline__ void stockham_stage(cx<Re> *lds, int tid,
   if constexpr (!LAST) __syncthreads();
 }
 
-template <int L, int S, int T, int STAGE, int NS, bool FROM_GLOBAL, bool TO_GLOBAL,
+template <int L, int S, int T, int STAGE, int NS, bool FROM_GLOBAL, bool TO_GLOBAL, int MAXR,
           class Re, class Load, class Store>
 __device__ __forceinline__ void stockham_run(cx<Re> *lds, int tid, const cx<Re> *__restrict__ tw,
                                              Load &load, Store &store) {
-  constexpr RadixList rl = make_radices(L);
+  constexpr RadixList rl = make_radices(L, MAXR);
   if constexpr (STAGE < rl.n) {
     constexpr int R = rl.r[STAGE];
     constexpr bool first = FROM_GLOBAL && STAGE == 0;
     constexpr bool last = TO_GLOBAL && STAGE == rl.n - 1;
     stockham_stage<L, R, NS, S, T, first, last>(lds, tid, tw, load, store);
-    stockham_run<L, S, T, STAGE + 1, NS * R, FROM_GLOBAL, TO_GLOBAL>(lds, tid, tw, load,
-                                                                      store);
+    stockham_run<L, S, T, STAGE + 1, NS * R, FROM_GLOBAL, TO_GLOBAL, MAXR>(lds, tid, tw, load,
+                                                                            store);
   }
 }
 
@@ -276,11 +284,12 @@ __device__ __forceinline__ void stockham_run(cx<Re> *lds, int tid, const cx<Re> 
 // (unnormalised).
 // TT threads work on the column (default FftShape<L>::T, one largest-radix
 // butterfly each; fewer threads take several butterflies per stage).
-template <int L, int S, bool FROM_GLOBAL, bool TO_GLOBAL, int TT = FftShape<L>::T, class Re,
-          class Load, class Store>
+// MAXR: the largest radix of the plan (8, or 4 for twice the threads).
+template <int L, int S, bool FROM_GLOBAL, bool TO_GLOBAL, int TT = FftShape<L>::T, int MAXR = 8,
+          class Re, class Load, class Store>
 __device__ __forceinline__ void fft_column(cx<Re> *lds, int tid, const cx<Re> *__restrict__ tw,
                                            Load &load, Store &store) {
-  stockham_run<L, S, TT, 0, 1, FROM_GLOBAL, TO_GLOBAL>(lds, tid, tw, load, store);
+  stockham_run<L, S, TT, 0, 1, FROM_GLOBAL, TO_GLOBAL, MAXR>(lds, tid, tw, load, store);
 }
 
 }  // namespace rism

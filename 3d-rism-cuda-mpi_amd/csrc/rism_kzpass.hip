@@ -14,8 +14,9 @@ namespace rism {
 // M-point DFT, G(m) = E(m) + w_m O(m) with E, O from Z(m) and Z(M-1-m) and
 // w_m = e^{-2 pi i (m+1/2)/nz}: the z-DFT at the positive half-shifted
 // frequencies.
-template <int M, int B, int TZ, class Re>
+template <int M, int B, int TZ, int MAXR, class Re>
 __global__ void __launch_bounds__(B *TZ)
+    __attribute__((amdgpu_waves_per_eu(MAXR == 4 ? 8 : 1)))
     k_zfwd_closure(const Re *__restrict__ t, const Re *__restrict__ u,
                    const Re *__restrict__ fr, Params P, cx<Re> *__restrict__ spec,
                    const cx<Re> *__restrict__ twM, const cx<Re> *__restrict__ zpre,
@@ -64,7 +65,7 @@ __global__ void __launch_bounds__(B *TZ)
     return cmul(C{cval(2 * p), cval(2 * p + 1)}, zpre[p]);
   };
   auto store = [&](int, C) {};
-  fft_column<M, -1, true, false, TZ>(col, j, twM, load, store);
+  fft_column<M, -1, true, false, TZ, MAXR>(col, j, twM, load, store);
 
   C *out = spec + ((size_t)v * M * P.ny + y) * P.nx + x;
   const size_t mstride = (size_t)P.ny * P.nx;
@@ -242,9 +243,10 @@ template <int M, class Re>
 static void launch_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<Re> *spec,
                         const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost,
                         hipStream_t st) {
-  constexpr int B = ZTile<M, Re>::B, TZ = ZTile<M, Re>::T;
-  hipLaunchKernelGGL((k_zfwd_closure<M, B, TZ, Re>), dim3(zgrid(P, B)), dim3(B * TZ), 0, st, t, u,
-                     fr, P, spec, twM, zpre, zpost);
+  using Z = ZTile<M, Re>;
+  constexpr int B = Z::BF, TZ = Z::TF;
+  hipLaunchKernelGGL((k_zfwd_closure<M, B, TZ, Z::MAXRF, Re>), dim3(zgrid(P, B)), dim3(B * TZ), 0,
+                     st, t, u, fr, P, spec, twM, zpre, zpost);
 }
 template <int M, int HM, class Re>
 static void launch_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
