@@ -815,9 +815,11 @@ int rebase_up(rism_ctx *c, int k0) {
 //     is never discarded, and the convergence test is the FP64 residual.
 // When the FP64 residual stops falling for kNoGain blocks it returns with
 // *fallback set and the state at the FP64 level, and the caller continues
-// with plain FP64 steps from there.
+// with plain FP64 steps from there.  A non-finite residual in phase 1 (the
+// FP32 exponent overflows above 88, FP64 only above 709) sets *restart: the
+// caller solves in FP64 from the initial t, which phase 1 left untouched.
 int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, bool &conver,
-                bool &diverge, int &low_steps, bool &fallback) {
+                int &low_steps, bool &fallback, bool &restart) {
   const bool verbose = c->opts.verbose != 0;
   const double hand_rms = std::max(tol, c->opts.mixed_switch > 0 ? c->opts.mixed_switch : 1e-3);
   const int block = c->opts.mixed_block > 0 ? c->opts.mixed_block : 24;
@@ -831,7 +833,10 @@ int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, b
     if ((rc = calc_step<float>(c, &rms))) return rc;
     ++low_steps;
     if (!std::isfinite(rms)) {
-      diverge = true;
+      if (verbose)
+        std::cout << " Step = " << istep << " FP32 overflow: FP64 from the initial Tuv"
+                  << std::endl;
+      restart = true;
       return RISM_OK;
     }
     if (verbose) std::cout << " Step = " << istep << " Reside = " << rms << " (FP32)" << std::endl;
@@ -853,10 +858,7 @@ int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, b
     ++istep;
     const int k0 = c->iT_cur;
     if ((rc = calc_step<double>(c, &rms))) return rc;
-    if (!std::isfinite(rms)) {
-      diverge = true;
-      return RISM_OK;
-    }
+    if (!std::isfinite(rms)) return RISM_OK;  // diverged: the caller's test sees rms
     if (verbose) std::cout << " Step = " << istep << " Reside = " << rms << std::endl;
     if (rms <= tol) {
       conver = true;
@@ -1336,11 +1338,15 @@ int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, do
   int istep = 0, last = 0, low_steps = 0;
   double rms = 0.0;
   if (c->opts.precision == 1 && c->prec == 1) {
-    bool fallback = false;
-    int rc = solve_mixed(c, maxstep, tol, istep, rms, conver, diverge, low_steps, fallback);
+    bool fallback = false, restart = false;
+    int rc = solve_mixed(c, maxstep, tol, istep, rms, conver, low_steps, fallback, restart);
     if (rc) return rc;
     last = istep;
-    if (fallback) {  // plain FP64 steps from the last FP64 evaluation on
+    diverge = !std::isfinite(rms) && !restart;
+    if (restart) {  // FP64 from the initial t (d.T[0]) with a fresh accelerator
+      c->prec = 0;
+      reset_an2(c);
+    } else if (fallback) {  // plain FP64 steps from the last FP64 evaluation on
       if ((rc = accel_step<double>(c))) return rc;
     }
   }

@@ -113,3 +113,23 @@ def test_mixed_mobley_known_answer(rism, oracle, gpu, golden):
     assert tm["status"] == rism.STATUS_CONVERGED
     assert abs(th["SFE_SC"] - ref["SFE_SC"]) < 0.05 * KCAL
     assert abs(th["PMV"] - ref["PMV"]) < 2e-5
+
+
+def test_mixed_fp32_overflow_restarts_in_fp64(rism, oracle, gpu, case24):
+    """HNC at 0.3x the charges: exp(-u + t) overflows within two steps (above
+    88 in FP32 and 709 in FP64).  When the plain FP32 phase overflows, the
+    mixed solve restarts in FP64 from the initial t with a fresh accelerator:
+    its outcome is the FP64 solve's, after the FP32 steps it has spent."""
+    sol, s, d, su, grid, box = case24
+    su = dict(su, uq=su["uq"] * 0.3)
+    out = []
+    for prec in (0, 1):
+        g = rism.RISM3D(grid, box, d, su, closure=rism.HNC, precision=prec)
+        th, tm = g.run(maxstep=500, tol=1e-6)
+        out.append(tm)
+        g.close()
+    fp64, mixed = out
+    print("FP64", fp64, "mixed", mixed)
+    assert mixed["steps_fp32"] >= 1
+    assert mixed["status"] == fp64["status"]
+    assert mixed["steps"] == mixed["steps_fp32"] + fp64["steps"]
