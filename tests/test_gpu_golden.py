@@ -126,3 +126,32 @@ def test_huv_cuv_dumps(rism, oracle, gpu, golden, tmp_path):
     text = open(tmp_path / "m.cuv0").read().splitlines()
     assert text[-1].strip().startswith("(") and text[-1].strip().endswith(",0)")
     g.close()
+
+
+def test_cli_tuv_checkpoint_restart(rism, oracle, gpu, golden, tmp_path):
+    """The CLI's ksave checkpoints (iterate.cu:26-35, 63-71): a run with ksave
+    writes <base>.tuv every ksave steps and at convergence; a second run of
+    the same input finds the file, uploads it as t and starts from the
+    converged state (first residual below the tolerance's order)."""
+    import gen_inputs as G
+    exe = os.path.join(ROOT, "3d-rism-cuda-mpi_amd", "rism3d")
+    shutil.copy(os.path.join(golden, "h2o_vv.xvk"), tmp_path / "h2o_vv.xvk")
+    sol = G.random_walk_solute(24, seed=1001, total_charge=0.0)
+    G.write_inp(tmp_path / "walk.inp", sol, "h2o_vv.xvk", (48, 48, 48), (24.0, 24.0, 24.0),
+                outlist="m", ksave=25)
+    runs = []
+    for _ in range(2):
+        p = subprocess.run([exe, "walk.inp"], cwd=tmp_path, capture_output=True, text=True,
+                           timeout=600)
+        assert p.returncode == 0, p.stdout + p.stderr
+        runs.append(p.stdout)
+    steps = [[float(l.split()[-1]) for l in out.splitlines() if l.startswith(" Step = ")]
+             for out in runs]
+    assert "saving Tuv to file:  walk.tuv  ..." in runs[0]
+    assert (tmp_path / "walk.tuv").exists()
+    assert len(steps[0]) > 25 and steps[0][-1] <= 1e-6
+    # restarted from the converged t (written at 6 significant digits)
+    assert steps[1][0] < 1e-4 * steps[0][0]
+    assert len(steps[1]) < len(steps[0]) // 2
+    a = oracle.parse_xmu(tmp_path / "walk.xmu")
+    assert np.isfinite(a["SFE_SC"])
