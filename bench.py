@@ -300,8 +300,11 @@ def bench_batch(args):
                                   "uxyz": xyz + shift}))
     log(f"[bench] batch: {nsol} solutes, {len(set(c[0] for c in cases))} grid shapes")
 
+    prec = 1 if args.precision == "mixed" else 0
+
     def make():
-        return [rism.RISM3D(g, b, solv, su, rism.KH, 2, 0.6, 0.2) for g, b, su in cases]
+        return [rism.RISM3D(g, b, solv, su, rism.KH, 2, 0.6, 0.2, precision=prec)
+                for g, b, su in cases]
 
     # warm-up (module load, first launches)
     w = make()[:4]
@@ -355,13 +358,14 @@ def bench_batch(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 bulk, f64 sums and FP64 steps" if prec else "f64",
         "data": "synthetic",
         "config": {
             "workload": ("batch: %d random-walk solutes (8-60 atoms, seeds 1004000+i, net "
                          "charge -1/0/+1) in TIP3P water (chi from the project's 1D DRISM-KH, "
                          "eps 78.4), boxes = extent + 12 A/side at 0.5 A, grids from "
-                         "{48..96}^3, KH, AN2 2 0.6 0.2, tol 1e-6, FP64" % nsol),
+                         "{48..96}^3, KH, AN2 2 0.6 0.2, tol 1e-6, %s"
+                         % (nsol, "mixed precision" if prec else "FP64")),
             "solves": nsol, "grid_shapes": len(set(c[0] for c in cases)),
             "batch_s": t_batch, "sequential_s": t_seq,
             "sequential_solves_per_s": nsol / t_seq,
@@ -398,7 +402,8 @@ def main():
                     help="history length of Anderson(m) mixing (0: the reference's AN2)")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
                     help="fp64: the reference's precision; mixed: FP32 bulk + FP64 sums and "
-                         "final FP64 steps; both: time both (value = FP64)")
+                         "FP64 steps; both: time both (value = FP64); the small-solute batch "
+                         "runs mixed contexts with mixed, FP64 ones otherwise")
     ap.add_argument("--beta", type=float, default=0.6,
                     help="mixing coefficient (AN2's m; beta of Anderson(m))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
