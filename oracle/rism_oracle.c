@@ -319,6 +319,7 @@ double or_splint(const double *xp, const double *yp, const double *coe, int np, 
  */
 int or_chi_shells(const double *xvv, int natv, int ntab, double dr, const double *ga,
                   int nga, double *chi) {
+  if (ntab < 1 || nga < 1) return 1;
   double dk = M_PI / (dr * ntab);
   double *ttab = (double *)malloc(sizeof(double) * ntab);
   for (int i = 0; i < ntab; ++i) ttab[i] = dk * (i + 1);
