@@ -197,12 +197,12 @@ struct rism_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   struct GraphKey {
-    int iT, iR, s0, s1, b;
+    int iT, iR, s0, s1, b, prec;
     double tol;
     const double *hist;
     bool operator==(const GraphKey &o) const {
-      return iT == o.iT && iR == o.iR && s0 == o.s0 && s1 == o.s1 && b == o.b && tol == o.tol &&
-             hist == o.hist;
+      return iT == o.iT && iR == o.iR && s0 == o.s0 && s1 == o.s1 && b == o.b &&
+             prec == o.prec && tol == o.tol && hist == o.hist;
     }
   } gkey{};
   bool have_step = false;
@@ -907,11 +907,10 @@ struct AsyncRun {
   bool finished = false;
   int status = RISM_STATUS_MAXSTEP, steps = 0;
   double rms = 0.0;
-  struct Log {
-    int iT;
-    double *R;
+  struct Log {  // the accelerator's indices when the step was enqueued
+    int iT, iR, s0, s1, b, count;
   };
-  std::vector<Log> log;  // per step: t index and residual buffer
+  std::vector<Log> log;  // per step
 };
 
 bool async_capable(const rism_ctx *c) {
@@ -923,7 +922,7 @@ int async_begin(AsyncRun &a, rism_ctx *c, int maxstep, double tol) {
   a.c = c;
   a.maxstep = maxstep;
   a.tol = tol;
-  a.log.assign(maxstep + 1, AsyncRun::Log{0, nullptr});
+  a.log.assign(maxstep + 1, AsyncRun::Log{0, 0, 0, 0, 0, 0});
   if (c->rms_cap < (size_t)maxstep + 1) {
     size_t cap = std::max<size_t>(1024, (size_t)maxstep + 1);
     int rc = c->alloc(&c->rms_hist, cap);
@@ -953,9 +952,10 @@ void advance_indices(rism_ctx *c) {
   c->binary = (b == 0) ? 1 : 0;
 }
 
+template <class Re>
 int ensure_graph(rism_ctx *c, double tol) {
   const rism_ctx::GraphKey key{c->iT_cur, c->iR_cur, c->iR_slot[0], c->iR_slot[1], c->binary,
-                               tol, c->rms_hist};
+                               c->prec, tol, c->rms_hist};
   if (c->gexec && key == c->gkey) return RISM_OK;
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -967,8 +967,8 @@ int ensure_graph(rism_ctx *c, double tol) {
   HIPCHECK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
   int rc = RISM_OK;
   for (int k = 0; k < kGraphSteps && !rc; ++k) {
-    rc = calc_step<double>(c, nullptr, false, tol);
-    if (!rc) rc = an2_step<double>(c);
+    rc = calc_step<Re>(c, nullptr, false, tol);
+    if (!rc) rc = an2_step<Re>(c);
   }
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c->st, &g);
@@ -989,28 +989,32 @@ int ensure_graph(rism_ctx *c, double tol) {
   return RISM_OK;
 }
 
-// enqueues up to `chunk` steps and the read-back of the flag: Picard steps
-// and remainders directly, whole periods of mixing steps as graph launches
+AsyncRun::Log index_log(const rism_ctx *c) {
+  return AsyncRun::Log{c->iT_cur, c->iR_cur, c->iR_slot[0], c->iR_slot[1], c->binary, c->count};
+}
+
+// enqueues up to `chunk` steps at level Re and the read-back of the flag:
+// Picard steps and remainders directly, whole periods of mixing steps as
+// graph launches
+template <class Re>
 int async_enqueue(AsyncRun &a, int chunk) {
   rism_ctx *c = a.c;
   int rc;
   for (int k = 0; k < chunk && a.istep < a.maxstep;) {
     if (c->count == 0 && k + kGraphSteps <= chunk && a.istep + kGraphSteps <= a.maxstep) {
-      if ((rc = ensure_graph(c, a.tol))) return rc;
+      if ((rc = ensure_graph<Re>(c, a.tol))) return rc;
       HIPCHECK(hipGraphLaunch(c->gexec, c->st));
       for (int j = 0; j < kGraphSteps; ++j) {
-        const int s = ++a.istep;
-        a.log[s] = AsyncRun::Log{c->iT_cur, c->d.R[c->iR_cur]};
+        a.log[++a.istep] = index_log(c);
         advance_indices(c);
       }
       k += kGraphSteps;
       continue;
     }
-    const int s = ++a.istep;
-    a.log[s] = AsyncRun::Log{c->iT_cur, c->d.R[c->iR_cur]};
-    if ((rc = calc_step<double>(c, nullptr, false, a.tol))) return rc;
+    a.log[++a.istep] = index_log(c);
+    if ((rc = calc_step<Re>(c, nullptr, false, a.tol))) return rc;
     c->fresh = false;
-    if ((rc = an2_step<double>(c))) return rc;
+    if ((rc = an2_step<Re>(c))) return rc;
     ++k;
   }
   HIPCHECK(hipMemcpyAsync(c->hflag, c->dflag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
@@ -1019,6 +1023,7 @@ int async_enqueue(AsyncRun &a, int chunk) {
 
 // after the stream has drained: finishes the run if the flag is up or the
 // step limit is reached, restoring the state of the deciding step
+template <class Re>
 int async_check(AsyncRun &a) {
   rism_ctx *c = a.c;
   if (c->ktime) ktime_resolve(c);
@@ -1033,14 +1038,23 @@ int async_check(AsyncRun &a) {
   HIPCHECK(hipStreamSynchronize(c->st));
   a.rms = c->hscal[0];
   c->async = false;
-  c->iT_last = a.log[k].iT;
-  c->d.Rlast = a.log[k].R;
-  c->last_prec = 0;
+  const AsyncRun::Log &L = a.log[k];
+  c->iT_last = L.iT;
+  lv<Re>(c).Rlast = lv<Re>(c).R[L.iR];
+  c->last_prec = c->prec;
   c->have_step = true;
   c->fresh = false;
-  // converged or diverged at k: no update followed (iterate.cu:50-72); at
-  // the step limit the last update did, as in the step-by-step loop
-  if (flag) c->iT_cur = a.log[k].iT;
+  // converged or diverged at k: no update followed (iterate.cu:50-72), so the
+  // accelerator stands where step k found it; at the step limit the last
+  // update did, as in the step-by-step loop
+  if (flag) {
+    c->iT_cur = L.iT;
+    c->iR_cur = L.iR;
+    c->iR_slot[0] = L.s0;
+    c->iR_slot[1] = L.s1;
+    c->binary = L.b;
+    c->count = L.count;
+  }
   return RISM_OK;
 }
 
@@ -1317,10 +1331,10 @@ int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, do
     AsyncRun a;
     int rc = async_begin(a, c, maxstep, tol);
     while (!rc && !a.finished) {
-      rc = async_enqueue(a, kAsyncChunk);
+      rc = async_enqueue<double>(a, kAsyncChunk);
       if (!rc && hipStreamSynchronize(c->st) != hipSuccess)
         rc = rism::set_error(RISM_ERR_HIP, "hipStreamSynchronize failed");
-      if (!rc) rc = async_check(a);
+      if (!rc) rc = async_check<double>(a);
     }
     c->async = false;
     if (rc) return rc;
@@ -1533,6 +1547,7 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
   std::vector<AsyncRun> runs(n);
   const int nt = std::max(1, std::min(threads > 0 ? threads : 1, n));
   std::vector<int> rcs(nt, RISM_OK);
+  std::vector<std::string> msgs(nt);  // error messages are per thread
   // worker w drives contexts w, w + nt, ...  FP64 AN2 contexts run without a
   // host round trip per step: enqueue a chunk on each of their streams, then
   // drain them in turn, until every run has finished.  Contexts whose
@@ -1552,14 +1567,14 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
       left = false;
       for (int i : fast) {
         if (rc) break;
-        if (!runs[i].finished) rc = async_enqueue(runs[i], kAsyncChunk);
+        if (!runs[i].finished) rc = async_enqueue<double>(runs[i], kAsyncChunk);
       }
       for (int i : fast) {
         if (rc) break;
         if (runs[i].finished) continue;
         if (hipStreamSynchronize(ctx[i]->st) != hipSuccess)
           rc = rism::set_error(RISM_ERR_HIP, "hipStreamSynchronize failed");
-        if (!rc) rc = async_check(runs[i]);
+        if (!rc) rc = async_check<double>(runs[i]);
         left = left || !runs[i].finished;
       }
     }
@@ -1569,13 +1584,14 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
       rc = rism_solve(ctx[i], maxstep, tol, &runs[i].status, &runs[i].steps, &runs[i].rms);
     }
     rcs[w] = rc;
+    if (rc) msgs[w] = rism_last_error();
   };
   std::vector<std::thread> pool;
   for (int w = 1; w < nt; ++w) pool.emplace_back(work, w);
   work(0);
   for (auto &th : pool) th.join();
   for (int w = 0; w < nt; ++w)
-    if (rcs[w]) return rcs[w];
+    if (rcs[w]) return rism::set_error(rcs[w], msgs[w]);
   for (int i = 0; i < n; ++i) {
     if (status) status[i] = runs[i].status;
     if (steps) steps[i] = runs[i].steps;
