@@ -460,7 +460,9 @@ constexpr int FKP = 16;
 __global__ void __launch_bounds__(256)
     k_fk_half(const double *__restrict__ atoms, int natu, int astride, Params P, double ubeta,
               cplx *__restrict__ fkraw) {
-  __shared__ double sa[ATILE * 4];
+  // per atom: x, y, z, q and the x-phase step e^{-i dkx x} (cos, sin), formed
+  // once per atom by the thread that stages it
+  __shared__ double sa[ATILE * 6];
   const int M = P.nz / 2;
   const int nxq = (P.nx + FKP - 1) / FKP;
   const size_t rows = (size_t)P.ny * M;
@@ -482,17 +484,21 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     for (int i = threadIdx.x; i < na; i += blockDim.x) {
       const double *src = atoms + (size_t)(a0 + i) * astride;
-      sa[4 * i] = src[0];
-      sa[4 * i + 1] = src[1];
-      sa[4 * i + 2] = src[2];
-      sa[4 * i + 3] = src[3];
+      double dsn, dcs;
+      sincos(dkx * src[0], &dsn, &dcs);
+      sa[6 * i] = src[0];
+      sa[6 * i + 1] = src[1];
+      sa[6 * i + 2] = src[2];
+      sa[6 * i + 3] = src[3];
+      sa[6 * i + 4] = dcs;
+      sa[6 * i + 5] = dsn;
     }
     __syncthreads();
     for (int a = 0; a < na; ++a) {
-      const double ax = sa[4 * a], ay = sa[4 * a + 1], az = sa[4 * a + 2], aq = sa[4 * a + 3];
-      double sn, cs, dsn, dcs;
+      const double ax = sa[6 * a], ay = sa[6 * a + 1], az = sa[6 * a + 2], aq = sa[6 * a + 3];
+      const double dcs = sa[6 * a + 4], dsn = sa[6 * a + 5];
+      double sn, cs;
       sincos(kx0 * ax + ky * ay + kz * az, &sn, &cs);
-      sincos(dkx * ax, &dsn, &dcs);
       // e = cos - i sin; step = dcs - i dsn
       double er = cs, ei = -sn;
 #pragma unroll
