@@ -58,6 +58,35 @@ struct alignas(16) V16 {
   Re a[K];
 };
 
+// 16-byte non-temporal accesses of the single-use streams of the updates
+template <class Re>
+struct NTVec;
+template <>
+struct NTVec<double> {
+  typedef double T __attribute__((ext_vector_type(2)));
+};
+template <>
+struct NTVec<float> {
+  typedef float T __attribute__((ext_vector_type(4)));
+};
+template <class Re>
+__device__ __forceinline__ V16<Re> ld_nt(const V16<Re> *p) {
+  using T = typename NTVec<Re>::T;
+  const T v = __builtin_nontemporal_load(reinterpret_cast<const T *>(p));
+  V16<Re> r;
+#pragma unroll
+  for (int k = 0; k < V16<Re>::K; ++k) r.a[k] = v[k];
+  return r;
+}
+template <class Re>
+__device__ __forceinline__ void st_nt(V16<Re> *p, const V16<Re> &x) {
+  using T = typename NTVec<Re>::T;
+  T v;
+#pragma unroll
+  for (int k = 0; k < V16<Re>::K; ++k) v[k] = x.a[k];
+  __builtin_nontemporal_store(v, reinterpret_cast<T *>(p));
+}
+
 template <class Re>
 __global__ void k_an2_picard(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
                              const V16<Re> *__restrict__ R, V16<Re> *__restrict__ Pb, Re mcoef,
@@ -87,15 +116,15 @@ __global__ void k_an2_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restri
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const Re s1 = (Re)scal[3], s2 = (Re)scal[4];
-  const V16<Re> ti = t[i], ri = R[i], p0 = P0[i], p1 = P1[i];
+  const V16<Re> ti = ld_nt(t + i), ri = ld_nt(R + i), p0 = ld_nt(P0 + i), p1 = ld_nt(P1 + i);
   V16<Re> a, q;
 #pragma unroll
   for (int k = 0; k < V16<Re>::K; ++k) {
     q.a[k] = ti.a[k] + mcoef * ri.a[k];
     a.a[k] = q.a[k] + s1 * (p0.a[k] - q.a[k]) + s2 * (p1.a[k] - q.a[k]);
   }
-  tnew[i] = a;
-  Pb[i] = q;
+  st_nt(tnew + i, a);
+  st_nt(Pb + i, q);
 }
 
 // End of a step: rms (cal_rms.cu:20-24), convergence / divergence flag
