@@ -197,21 +197,22 @@ __global__ void k_andm_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restr
                               Re beta, size_t nv) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
-  const V16<Re> ti = t[i], ri = R[i];
+  const V16<Re> ti = ld_nt(t + i), ri = ld_nt(R + i);
   V16<Re> q, a;
 #pragma unroll
   for (int k = 0; k < V16<Re>::K; ++k) a.a[k] = q.a[k] = ti.a[k] + beta * ri.a[k];
 #pragma unroll
   for (int h = 0; h < RISM_MAX_HIST; ++h) {
     if (h < H.nh) {
-      const V16<Re> p = reinterpret_cast<const V16<Re> *>(H.p[h])[i];
+      // Pb may be the oldest P_h: this thread reads element i before it writes it
+      const V16<Re> p = ld_nt(reinterpret_cast<const V16<Re> *>(H.p[h]) + i);
       const Re x = (Re)H.x[h];
 #pragma unroll
       for (int k = 0; k < V16<Re>::K; ++k) a.a[k] = fma(x, p.a[k] - q.a[k], a.a[k]);
     }
   }
-  tnew[i] = a;
-  Pb[i] = q;
+  st_nt(tnew + i, a);
+  st_nt(Pb + i, q);
 }
 
 // the closure exponent -u + t, formed in FP64 and rounded once
