@@ -1923,6 +1923,44 @@ int rism_k_get_fk(rism_ctx *c, double *out) {
   return RISM_OK;
 }
 
+int rism_k_copy_bench(size_t bytes, int reps, double *avg_ms) {
+  if (bytes < 16 || reps < 1) return rism::set_error(RISM_ERR_ARG, "bytes >= 16 and reps >= 1");
+  const size_t n = (bytes / 16) * 2;  // doubles, whole 16-byte vectors
+  double *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = RISM_OK;
+  if (hipMalloc((void **)&a, n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&b, n * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    rc = rism::set_error(RISM_ERR_NOMEM, "hipMalloc failed in rism_k_copy_bench");
+  }
+  if (!rc && (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+              hipEventCreate(&e1) != hipSuccess))
+    rc = rism::set_error(RISM_ERR_HIP, "stream/event creation failed");
+  if (!rc && (hipMemsetAsync(a, 0, n * sizeof(double), st) != hipSuccess ||
+              rism::run_copy(a, b, n, st) != hipSuccess))  // first touch, untimed
+    rc = rism::set_error(RISM_ERR_HIP, "copy warm-up failed");
+  if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = RISM_ERR_HIP;
+  for (int r = 0; r < reps && !rc; ++r)
+    if (rism::run_copy(a, b, n, st) != hipSuccess) rc = RISM_ERR_HIP;
+  if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+    rc = RISM_ERR_HIP;
+  if (!rc) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (avg_ms) *avg_ms = ms / reps;
+  } else if (rc == RISM_ERR_HIP) {
+    rism::set_error(rc, "copy launch failed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 int rism_k_an2_state(rism_ctx *c, double *x, double *s, int *count, int *binary) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
   if (x) {

@@ -127,6 +127,8 @@ hipError_t run_rebase_up(double *out, const double *t0, const float *d, size_t n
                          hipStream_t st);
 hipError_t run_rebase_down(const double *x, const double *t0, float *out, size_t n,
                            hipStream_t st);
+// streaming copy of n doubles (n even): reads n, writes n
+hipError_t run_copy(const double *in, double *out, size_t n, hipStream_t st);
 hipError_t run_potential_r(const double *atoms, int natu, const Params &P, double ikbt,
                            double ubeta, int do_coul, double *u, double *fr, hipStream_t st);
 hipError_t run_fk_half(const double *atoms, int natu, int astride, const Params &P,

@@ -1,5 +1,7 @@
 // rism_kmisc.hip -- FP64 reductions, the AN2 update, set-up potentials,
 // k-space Coulomb, initial t, derived fields and thermodynamic partial sums.
+#include <algorithm>
+
 #include "rism_common.h"
 
 namespace rism {
@@ -647,6 +649,25 @@ __global__ void k_apply_phase(cplx *__restrict__ s, Params P, int inverse) {
   const cplx a = s[i];
   // phi = -i sg : (x, y) -> sg * (y, -x); conj(phi) = i sg : sg * (-y, x)
   s[i] = inverse ? mk(-sg * a.y, sg * a.x) : mk(sg * a.y, -sg * a.x);
+}
+
+// Plain streaming copy of n 16-byte vectors: the reference point for the HBM
+// traffic counters and for the achievable copy bandwidth (reads n, writes n).
+__global__ void __launch_bounds__(256)
+    k_copy_stream(const V16<double> *__restrict__ in, V16<double> *__restrict__ out, size_t nv) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
+    st_nt(out + i, ld_nt(in + i));
+}
+
+hipError_t run_copy(const double *in, double *out, size_t n, hipStream_t st) {
+  if (n % 2) return hipErrorInvalidValue;
+  const size_t nv = n / 2;
+  // 256 CUs x 8 workgroups, grid-stride beyond that
+  const unsigned blocks = (unsigned)std::min<size_t>(nblk(nv, 256), 2048);
+  hipLaunchKernelGGL(k_copy_stream, dim3(blocks), dim3(256), 0, st,
+                     (const V16<double> *)in, (V16<double> *)out, nv);
+  return hipGetLastError();
 }
 
 hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, double *out,

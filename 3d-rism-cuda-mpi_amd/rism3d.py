@@ -43,7 +43,7 @@ EXPORTS = [
     "rism_k_fft_c2r", "rism_k_get_fk", "rism_k_an2_state", "rism_set_kernel_timing",
     "rism_get_kernel_stats", "rism_set_precision", "rism_get_precision",
     "rism_solve_batch", "rism_run_batch", "rism_write_huv", "rism_write_cuv",
-    "rism_analysis",
+    "rism_analysis", "rism_k_copy_bench",
 ]
 KERNELS = ["zfwd_closure", "ypass_fwd", "xpass_oz", "ypass_inv", "zinv_resid", "reduce",
            "an2_update", "potential_r", "fk_half", "thermo_partial"]
@@ -177,6 +177,7 @@ def lib():
     L.rism_k_an2_state.argtypes = [vp, _dp, _dp, pi, pi]
     L.rism_set_kernel_timing.argtypes = [vp, ci]
     L.rism_get_kernel_stats.argtypes = [vp, ctypes.POINTER(KStat)]
+    L.rism_k_copy_bench.argtypes = [ctypes.c_size_t, ci, _dp]
     _lib = L
     return L
 
@@ -272,6 +273,13 @@ def fft_c2r(F, box):
     b = _f64(box)
     _check(lib().rism_k_fft_c2r(nx, ny, nz, _ptr(b), F.ctypes.data_as(_dp), _ptr(out)))
     return out
+
+
+def copy_bench(nbytes, reps=5):
+    """Average ms of one plain device copy of nbytes (rism_k_copy_bench)."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().rism_k_copy_bench(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 # ---------------------------------------------------------------------------

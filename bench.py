@@ -1,32 +1,54 @@
 """Benchmark: seconds per converged 3D-RISM solve on one MI355X.
 
-Workload (default "cfg2"): 512^3 grid in a 256 A box; a synthetic 12877-atom
-protein-like cluster (uniform in a 30 A sphere, >= 1.0 A apart, C/N/O/H + 1 %
-S, charges ~ N(0, 0.35 e) neutralised to +2 e, seed 1002); 0.2 M NaCl in
-TIP3P water (chi_vv from the project's 1D DRISM-KH at eps 78.4, 298.15 K,
-tests/golden/tip3p_nacl02_drism.xvk, 4 sites); KH closure; AN2 2 0.6 0.2;
-tolerance 1e-6; FP64 throughout.  One step = one full solve:
+Headline workload (default): a 512^3 grid in a 256 A box; a synthetic
+12877-atom protein-like cluster (uniform in a 30 A sphere, >= 1.0 A apart,
+C/N/O/H + 1 % S, charges ~ N(0, 0.35 e) neutralised to +2 e, seed 1002);
+0.2 M NaCl in TIP3P water (chi_vv from the project's 1D DRISM-KH at eps 78.4,
+298.15 K, tests/golden/tip3p_nacl02_drism.xvk, 4 sites); KH closure; AN2
+2 0.6 0.2; tolerance 1e-6; FP64 throughout.  One step = one full solve:
 LJ/Coulomb potentials, the initial t, the fixed-point loop to convergence and
 the thermodynamics, with the solute and the chi table already resident in
-HBM when the clock starts.
+HBM when the clock starts.  The timed solves run the production path (no
+per-kernel instrumentation).
 
-Also reported, per the roofline contract: the dominant hot-loop kernel's
-achieved HBM bandwidth (its algorithmic bytes per launch / its average
-hipEvent-timed launch duration, measured on the solver's own stream during
-the timed solves), and an FP64 CPU baseline (the oracle, a plain-C OpenMP
-restatement of the reference algorithm) timed on a bounded sample on the
-host cores and extrapolated to one solve.
+The run is held to a wall-clock budget (--budget, default 450 s from process
+start): warm-up and timed solves stop early when the next one would not fit,
+and the line reports the counts actually run.  After the headline it adds,
+while the budget lasts:
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--workload cfg2|cfg1|cfg3|mobley|batch|batch-medium]
+* roofline: the dominant hot-loop kernel's algorithmic bytes per launch over
+  its average hipEvent-timed duration on the solver's stream, from a separate
+  instrumented solve of the same input; traffic from two rocprofv3 --pmc
+  child runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a plain
+  copy of known size;
+* mixed: the mixed-precision solve of the same input (s/solve, speed-up,
+  |dmu|, dPMV and rms(dh) against the FP64 solve);
+* batch: 256 small solutes solved as one batch against the same solves one
+  after another;
+* protein_test2: the reference's own 512^3 RBD-ACE2 input
+  (example-cygnus/test2.inp, naclaq_vv.xvk), the case its published timings
+  are quoted on;
+* cpu_baseline: the FP64 oracle on the host cores (a bounded sample scaled to
+  one solve, plus a full 64^3 solve);
+* anderson10: the configuration's Anderson(10) accelerator, FP64 and mixed.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--budget S]
+                  [--workload cfg2|cfg1|cfg3|mobley|test2|batch|batch-medium]
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import threading
 import time
 
-import numpy as np
+T_START = time.time()
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
@@ -35,10 +57,62 @@ sys.path.insert(0, os.path.join(ROOT, "3d-rism-cuda-mpi_amd"))
 METRIC = ("s/solve at 512^3, batched solves/s; mixed vs FP64 at "
           "|Δμ|<0.05 kcal/mol")
 HBM_PEAK_GBS = 8000.0
+KCAL = 4184.0
+# the reference's published time per solve of example-cygnus/test2.inp
+# (README.md:25: 1 node, 4 GPUs, GPUDirect)
+REF_PUBLISHED_TEST2_S = 413.9
+# example-pegasus/test2.xmu:2,8,14 (an older build of the reference)
+REF_TEST2_XMU = {"SFE_SC": 23324102.50402, "SFE_GF": 6329885.06176, "PMV": 69.89044}
 
 
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print(f"[bench {time.time() - T_START:6.1f}s]", *a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# device plumbing (HIP runtime through ctypes; the solver owns its streams)
+# ---------------------------------------------------------------------------
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        import ctypes
+        for name in ("libamdhip64.so", "libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise SystemExit("bench.py needs the HIP runtime (libamdhip64.so)")
+    return _hip
+
+
+def device_count():
+    import ctypes
+    n = ctypes.c_int(0)
+    if hip().hipGetDeviceCount(ctypes.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+def sync():
+    rc = hip().hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def centred(sol):
+    """solute.cu:18-48 centring (integer-A shift of the extent's midpoint)."""
+    xyz = np.asarray(sol["xyz"], float)
+    mn, mx = xyz.min(0), xyz.max(0)
+    shift = np.array([np.floor(-(mx[i] - mn[i]) / 2 - mn[i] + 0.5) for i in range(3)])
+    return dict(sol, xyz=xyz + shift)
 
 
 def workload(name):
@@ -46,76 +120,479 @@ def workload(name):
     import __graft_entry__ as ge
     rism = ge.load_rism3d()
     if name == "cfg2":
-        sol = G.cfg2_solute()
+        sol = centred(G.cfg2_solute())
         xvk = "tip3p_nacl02_drism.xvk"
         grid, box = (512, 512, 512), (256.0, 256.0, 256.0)
         desc = ("cfg2: 512^3 grid / 256 A box, synthetic 12877-atom cluster (R 30 A, +2 e, "
                 "seed 1002), 0.2 M NaCl in TIP3P (chi from the project's 1D DRISM-KH, "
                 "eps 78.4, 298.15 K, 4 sites), KH, AN2 2 0.6 0.2, tol 1e-6, FP64")
     elif name == "cfg1":
-        sol = G.cfg1_solute()
+        sol = centred(G.cfg1_solute())
         xvk = "tip3p_drism.xvk"
         grid, box = (64, 64, 64), (32.0, 32.0, 32.0)
         desc = ("cfg1: 64^3 / 32 A, 24-atom random-walk solute (seed 1001), TIP3P water "
                 "(chi from the project's 1D DRISM-KH, eps 78.4, 298.15 K), KH, AN2 2 0.6 0.2, "
                 "tol 1e-6, FP64")
     elif name == "cfg3":
-        sol = G.cfg3_solute()
+        sol = centred(G.cfg3_solute())
         xvk = "tip3p_kcl01_drism.xvk"
         grid, box = (256, 256, 256), (128.0, 128.0, 128.0)
         desc = ("cfg3: 256^3 / 128 A, synthetic 3000-atom cluster (R 19 A, 0 e, seed 1003), "
                 "0.1 M KCl in TIP3P (chi from the project's 1D DRISM-KH, eps 78.4, 298.15 K, "
                 "4 sites), tol 1e-6, FP64")
-    elif name == "mobley":
-        d = rism.read_input(os.path.join(GOLDEN, "3drism.dat"),
-                            os.path.join(GOLDEN, "mobley_3053621"), solvent_dir=GOLDEN)
+    elif name in ("mobley", "test2"):
+        if name == "mobley":
+            d = rism.read_input(os.path.join(GOLDEN, "3drism.dat"),
+                                os.path.join(GOLDEN, "mobley_3053621"), solvent_dir=GOLDEN)
+            desc = "example_milc mobley_3053621: 256^3 / 64 A, 12 atoms, ctip3p.xvk, KH, FP64"
+        else:
+            d = rism.read_input(os.path.join(GOLDEN, "cygnus_test2.inp"), solvent_dir=GOLDEN)
+            desc = ("example-cygnus/test2.inp: RBD-ACE2, 12877 atoms, 512^3 / 256 A, "
+                    "naclaq_vv.xvk, KH, AN2 2 0.6 0.2, tol 1e-6, FP64")
         sol = {"sig": d["usig"], "eps": d["ueps"], "q": d["uq"], "xyz": d["uxyz"]}
-        xvk = "ctip3p.xvk"
-        grid, box = (256, 256, 256), (64.0, 64.0, 64.0)
-        desc = "example_milc mobley_3053621: 256^3 / 64 A, 12 atoms, ctip3p.xvk, KH, FP64"
+        xvk = os.path.basename(d["fsolvent"])
+        grid, box = tuple(int(g) for g in d["grid"]), tuple(float(b) for b in d["box"])
     else:
         raise SystemExit(f"unknown workload {name}")
-    if name != "mobley":
-        from_centre = sol["xyz"].copy()
-        mn, mx = from_centre.min(0), from_centre.max(0)
-        shift = np.array([np.floor(-(mx[i] - mn[i]) / 2 - mn[i] + 0.5) for i in range(3)])
-        sol = dict(sol, xyz=from_centre + shift)  # solute.cu:18-48 centring
     solv = rism.read_xvk(os.path.join(GOLDEN, xvk))
     return rism, grid, box, solv, sol, xvk, desc
 
 
-# HBM bytes per launch of the hot-loop kernels at cfg2 (512^3, 4 sites, FP64),
-# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one counter per
-# run) and scaled by the same counters on a plain 2 GiB device copy
-# (fetch x2, write x1).
-MEASURED_TRAFFIC_CFG2 = {
-    "zfwd_closure": 13.97e9, "ypass_fwd": 8.59e9, "xpass_oz": 11.73e9, "ypass_inv": 8.59e9,
-    "zinv_resid": 24.35e9, "an2_update": 25.77e9,
-}
+def solu_dict(sol):
+    return {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"], "uxyz": sol["xyz"]}
 
 
-def algorithmic_bytes(natv, N, hist=2):
-    """Minimal HBM bytes per launch of each hot-loop kernel (FP64 words of 8 B)
-    with `hist` history residuals (2 for AN2, m for Anderson(m)):
+def algorithmic_bytes(natv, N, hist=2, w=8):
+    """Minimal HBM bytes per launch of each hot-loop kernel (w bytes per bulk
+    word) with `hist` history residuals (2 for AN2, m for Anderson(m)):
     zfwd reads t, u (2n) and fr (1), writes the half spectrum (n);
     each y pass reads and writes the spectrum (2n);
     xpass_oz reads the spectrum (n) and the half fk (1), writes (n);
     zinv reads the spectrum, t, u (3n) and the history residuals (hist n),
     writes R (n);
     the update reads t, R and the history P_p (hist n), writes t', P_new (2n)."""
-    w = 8.0 * N
+    ww = float(w) * N
     n = natv
     return {
-        "zfwd_closure": (3 * n + 1) * w,
-        "ypass_fwd": 2 * n * w,
-        "xpass_oz": (2 * n + 1) * w,
-        "ypass_inv": 2 * n * w,
-        "zinv_resid": (4 + hist) * n * w,
-        "an2_update": (4 + hist) * n * w,
+        "zfwd_closure": (3 * n + 1) * ww,
+        "ypass_fwd": 2 * n * ww,
+        "xpass_oz": (2 * n + 1) * ww,
+        "ypass_inv": 2 * n * ww,
+        "zinv_resid": (4 + hist) * n * ww,
+        "an2_update": (4 + hist) * n * ww,
     }
 
 
-def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv, budget_s=30.0):
+def fused_bytes(natv, N, m, w):
+    """The fused lower bound of one iteration, each 3D transform counted as
+    one HBM round trip: N [(12 + 3 m) n + 2] w bytes, m history pairs (2 for
+    AN2), w bytes per bulk value."""
+    return N * ((12 + 3 * m) * natv + 2) * w
+
+
+class Budget:
+    def __init__(self, total):
+        self.end = T_START + total
+
+    def left(self):
+        return self.end - time.time()
+
+
+# ---------------------------------------------------------------------------
+# the phases of the default run
+# ---------------------------------------------------------------------------
+def timed_solves(g, warmup, steps, warm_cap_s, timed_cap_s, maxstep, label):
+    """Warm-up solves (at least one, at most `warmup`, within warm_cap_s) then
+    timed solves (at least one, at most `steps`, stopping when the next would
+    end beyond timed_cap_s), each a full rism_run on the production path.
+    Returns (warm-ups run, timed solves run, wall s, [(thermo, timing)])."""
+    t0 = time.time()
+    nw = 0
+    per = None
+    while nw < max(1, warmup):
+        if nw >= 1 and time.time() - t0 + per > warm_cap_s:
+            break
+        ts = time.time()
+        th, tm = g.run(maxstep=maxstep, tol=1e-6)
+        per = time.time() - ts
+        nw += 1
+        log(f"{label} warm-up {nw}: {per:.2f} s, {tm['steps']} steps, status {tm['status']}")
+    sync()
+    out = []
+    t1 = time.perf_counter()
+    while len(out) < max(1, steps):
+        el = time.perf_counter() - t1
+        if out and el + el / len(out) > timed_cap_s:
+            break
+        th, tm = g.run(maxstep=maxstep, tol=1e-6)
+        out.append((th, tm))
+        log(f"{label} timed {len(out)}: {tm['steps']} steps, SFE_SC {th['SFE_SC']:.5f}")
+    sync()
+    wall = time.perf_counter() - t1
+    return nw, len(out), wall, out
+
+
+def kernel_profile(g, maxstep):
+    """One instrumented run (hipEvents around every hot-path launch, which
+    runs the step-by-step loop): {kernel: (launches, total ms)}."""
+    g.kernel_timing(True)
+    g.run(maxstep=maxstep, tol=1e-30)
+    ks = g.kernel_stats()
+    g.kernel_timing(False)
+    return {k: v for k, v in ks.items() if v[0] > 0}
+
+
+def per_kernel_table(ks, ab):
+    return {k: {"launches": v[0], "avg_ms": v[1] / v[0],
+                "GB/s": (ab[k] / (v[1] / v[0] / 1000.0) / 1e9) if k in ab else None,
+                "algorithmic_GB": ab[k] / 1e9 if k in ab else None}
+            for k, v in ks.items()}
+
+
+def phase_headline(args, budget, R):
+    rism, grid, box, solv, sol, xvk, desc = workload(args.workload)
+    natv = solv["natv"]
+    N = grid[0] * grid[1] * grid[2]
+    am = args.anderson_m
+    clos = closure_code(rism, args, "PSE3" if args.workload == "cfg3" else "KH")
+    log(f"workload {args.workload}: grid {grid}, natu {len(sol['sig'])}, natv {natv}")
+    g = rism.RISM3D(grid, box, solv, solu_dict(sol), clos, 2, args.beta, 0.2, anderson_m=am)
+    nw, nt, wall, res = timed_solves(g, args.warmup, args.steps, args.warm_cap, args.timed_cap,
+                                     args.maxstep, "fp64")
+    th, tm = res[-1]
+    if tm["status"] != rism.STATUS_CONVERGED:
+        log(f"WARNING: solve did not converge (status {tm['status']})")
+    s_per = wall / nt
+    iters = tm["steps"]
+    mh = am if am > 0 else 2
+    iter_ms = tm["iterate_ms"] / max(iters, 1)
+    R["out"].update({
+        "value": s_per, "steps": nt, "warmup": nw, "ms_per_step": 1000.0 * s_per,
+    })
+    R["out"]["config"].update({
+        "workload": desc, "grid": list(grid), "box": list(box), "natu": int(len(sol["sig"])),
+        "natv": int(natv), "closure": args.closure or ("PSE3" if args.workload == "cfg3"
+                                                        else "KH"),
+        "anderson_m": am, "iterations": iters, "status": tm["status"],
+        "setup_ms": tm["setup_ms"], "iterate_ms": tm["iterate_ms"],
+        "thermo_ms": tm["thermo_ms"], "ms_per_iteration": iter_ms,
+        "iteration_fused_bound_GB": fused_bytes(natv, N, mh, 8) / 1e9,
+        "iteration_fused_bound_frac": fused_bytes(natv, N, mh, 8) / (iter_ms / 1000.0)
+        / (HBM_PEAK_GBS * 1e9),
+        "SFE_SC": th["SFE_SC"], "SFE_GF": th["SFE_GF"], "PMV": th["PMV"],
+        "solves_s": [r[1]["setup_ms"] / 1e3 + r[1]["iterate_ms"] / 1e3 + r[1]["thermo_ms"] / 1e3
+                     for r in res],
+        "steps_requested": args.steps, "warmup_requested": args.warmup,
+    })
+    log(f"fp64: {s_per:.3f} s/solve over {nt} solves ({iters} iterations, "
+        f"{iter_ms:.2f} ms/iteration)")
+    ctx = {"rism": rism, "grid": grid, "box": box, "solv": solv, "sol": sol, "xvk": xvk,
+           "natv": natv, "N": N, "clos": clos, "am": am, "th": th, "tm": tm, "iters": iters}
+    # h of the FP64 solve for rms(dh) of the mixed one, and one instrumented
+    # run for the per-kernel averages
+    if args.precision in ("mixed", "both"):
+        ctx["h64"] = g.fields(rism.FIELD_H)
+    ks = kernel_profile(g, args.profile_steps)
+    g.close()
+    ab = algorithmic_bytes(natv, N, mh)
+    R["out"]["config"]["kernels"] = per_kernel_table(ks, ab)
+    hot = {k: v for k, v in ks.items() if k in ab}
+    dom = max(hot, key=lambda k: hot[k][1])
+    avg_s = hot[dom][1] / hot[dom][0] / 1000.0
+    achieved = ab[dom] / avg_s / 1e9
+    R["out"]["roofline"] = {
+        "kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": avg_s * 1000.0,
+        "source": ("hipEvents on the solver's stream around every launch of an instrumented "
+                   f"run of the same input ({hot[dom][0]} launches)"),
+    }
+    # setup kernels: pair evaluations per second
+    natu = len(sol["sig"])
+    if "potential_r" in ks:
+        pot_ms = ks["potential_r"][1] / ks["potential_r"][0]
+        R["out"]["config"]["potential_pairs_per_s"] = N * natu / (pot_ms / 1e3)
+    ctx["dom"] = dom
+    return ctx
+
+
+def phase_mixed(args, budget, R, ctx):
+    rism = ctx["rism"]
+    gm = rism.RISM3D(ctx["grid"], ctx["box"], ctx["solv"], solu_dict(ctx["sol"]), ctx["clos"],
+                     2, args.beta, 0.2, anderson_m=ctx["am"], precision=1)
+    nw, nt, wall, res = timed_solves(gm, 1, args.steps, 0.0, args.mixed_cap, args.maxstep,
+                                     "mixed")
+    thm, tmm = res[-1]
+    rms_dh = None
+    if ctx.get("h64") is not None:
+        h32 = gm.fields(rism.FIELD_H)
+        h32 -= ctx["h64"]
+        rms_dh = float(np.sqrt(np.mean(h32 * h32)))
+        del h32
+    ksm = kernel_profile(gm, args.profile_steps)
+    gm.close()
+    th = ctx["th"]
+    dmu = thm["SFE_SC"] - th["SFE_SC"]
+    s64 = R["out"]["value"]
+    N, natv = ctx["N"], ctx["natv"]
+    mh = ctx["am"] if ctx["am"] > 0 else 2
+    ab32 = algorithmic_bytes(natv, N, mh, w=4)
+    R["out"]["mixed"] = {
+        "s_per_solve": wall / nt, "solves_timed": nt, "warmup": nw,
+        "speedup_vs_fp64": s64 / (wall / nt),
+        "iterations": tmm["steps"], "iterations_fp32": tmm["steps_fp32"],
+        "status": tmm["status"], "setup_ms": tmm["setup_ms"], "iterate_ms": tmm["iterate_ms"],
+        "SFE_SC": thm["SFE_SC"], "dSFE_SC_J_per_mol": dmu, "dSFE_SC_kcal_per_mol": dmu / KCAL,
+        "gate_0.05_kcal_per_mol": bool(abs(dmu) <= 0.05 * KCAL),
+        "dPMV": thm["PMV"] - th["PMV"], "rms_dh": rms_dh,
+        "iteration_fused_bound_frac": ((tmm["steps_fp32"] * fused_bytes(natv, N, mh, 4)
+                                        + (tmm["steps"] - tmm["steps_fp32"])
+                                        * fused_bytes(natv, N, mh, 8))
+                                       / (tmm["iterate_ms"] / 1000.0) / (HBM_PEAK_GBS * 1e9)),
+        "fp32_kernels": per_kernel_table(ksm, ab32),
+    }
+    log(f"mixed: {wall / nt:.3f} s/solve, {s64 / (wall / nt):.2f}x, dSFE {dmu:.3f} J/mol")
+
+
+def phase_anderson(args, budget, R, ctx, m=10):
+    rism = ctx["rism"]
+    out = {}
+    for prec, label in ((0, "fp64"), (1, "mixed")):
+        if budget.left() < 60:
+            out[label] = "skipped: budget"
+            break
+        g = rism.RISM3D(ctx["grid"], ctx["box"], ctx["solv"], solu_dict(ctx["sol"]), ctx["clos"],
+                        2, args.beta, 0.2, anderson_m=m, precision=prec)
+        sync()
+        t0 = time.perf_counter()
+        th, tm = g.run(maxstep=args.maxstep, tol=1e-6)
+        sync()
+        wall = time.perf_counter() - t0
+        g.close()
+        out[label] = {"s_per_solve": wall, "iterations": tm["steps"], "status": tm["status"],
+                      "iterations_fp32": tm["steps_fp32"], "SFE_SC": th["SFE_SC"],
+                      "ms_per_iteration": tm["iterate_ms"] / max(1, tm["steps"])}
+        log(f"anderson({m}) {label}: {wall:.2f} s, {tm['steps']} steps")
+    if isinstance(out.get("fp64"), dict) and isinstance(out.get("mixed"), dict):
+        d = out["mixed"]["SFE_SC"] - out["fp64"]["SFE_SC"]
+        out["mixed_speedup_vs_fp64"] = out["fp64"]["s_per_solve"] / out["mixed"]["s_per_solve"]
+        out["dSFE_SC_kcal_per_mol"] = d / KCAL
+    R["out"]["anderson%d" % m] = out
+
+
+def phase_test2(args, budget, R):
+    """The reference's own production input, timed beside the synthetic one."""
+    rism, grid, box, solv, sol, xvk, desc = workload("test2")
+    out = {"workload": desc}
+    for prec, label in ((0, "fp64"), (1, "mixed")):
+        if budget.left() < 45:
+            out[label] = "skipped: budget"
+            break
+        g = rism.RISM3D(grid, box, solv, solu_dict(sol), rism.KH, 2, 0.6, 0.2, precision=prec)
+        sync()
+        t0 = time.perf_counter()
+        th, tm = g.run(maxstep=args.maxstep, tol=1e-6)
+        sync()
+        wall = time.perf_counter() - t0
+        g.close()
+        out[label] = {"s_per_solve": wall, "iterations": tm["steps"], "status": tm["status"],
+                      "iterations_fp32": tm["steps_fp32"], "setup_ms": tm["setup_ms"],
+                      "SFE_SC": th["SFE_SC"], "SFE_GF": th["SFE_GF"], "PMV": th["PMV"]}
+        log(f"test2 {label}: {wall:.2f} s, {tm['steps']} steps, SFE_SC {th['SFE_SC']:.3f}")
+    if isinstance(out.get("fp64"), dict):
+        out["reference_published_s"] = REF_PUBLISHED_TEST2_S
+        out["reference_published_hw"] = "4 GPUs of one Cygnus node, GPUDirect (README.md:25)"
+        out["speedup_vs_reference_published"] = REF_PUBLISHED_TEST2_S / out["fp64"]["s_per_solve"]
+        out["reference_xmu_older_build"] = REF_TEST2_XMU
+    if isinstance(out.get("mixed"), dict) and isinstance(out.get("fp64"), dict):
+        d = out["mixed"]["SFE_SC"] - out["fp64"]["SFE_SC"]
+        out["mixed_speedup_vs_fp64"] = out["fp64"]["s_per_solve"] / out["mixed"]["s_per_solve"]
+        out["mixed_dSFE_SC_kcal_per_mol"] = d / KCAL
+    R["out"]["protein_test2"] = out
+
+
+def batch_cases(rism, n):
+    import gen_inputs as G
+    cases = []
+    for i in range(n):
+        sol = G.batch_solute(i)
+        grid, box = G.batch_grid(sol["xyz"])
+        cases.append((grid, box, solu_dict(centred(sol))))
+    return cases
+
+
+def phase_batch(args, budget, R, nsol=256, prec=0, stepwise=True):
+    import __graft_entry__ as ge
+    rism = ge.load_rism3d()
+    solv = rism.read_xvk(os.path.join(GOLDEN, "tip3p_drism.xvk"))
+    cases = batch_cases(rism, nsol)
+
+    def make():
+        return [rism.RISM3D(g, b, solv, su, rism.KH, 2, 0.6, 0.2, precision=prec)
+                for g, b, su in cases]
+
+    w = make()[:4]  # warm-up (first launches of the small-grid kernels)
+    rism.run_batch(w, threads=args.batch_threads)
+    for g in w:
+        g.close()
+    gs = make()
+    sync()
+    t0 = time.perf_counter()
+    out = rism.run_batch(gs, threads=args.batch_threads)
+    sync()
+    t_batch = time.perf_counter() - t0
+    for g in gs:
+        g.close()
+    steps = np.array([o[2] for o in out])
+    sfe_batch = [o[0]["SFE_SC"] for o in out]
+    gs = make()
+    sync()
+    t0 = time.perf_counter()
+    sfe_seq = [g.run()[0]["SFE_SC"] for g in gs]
+    sync()
+    t_seq = time.perf_counter() - t0
+    for g in gs:
+        g.close()
+    res = {
+        "workload": ("%d random-walk solutes (8-60 atoms, seeds 1004000+i, net charge -1/0/+1) "
+                     "in TIP3P water (chi from the project's 1D DRISM-KH, eps 78.4), boxes = "
+                     "extent + 12 A/side at 0.5 A, grids from {48..96}^3, KH, AN2 2 0.6 0.2, "
+                     "tol 1e-6, %s" % (nsol, "mixed precision" if prec else "FP64")),
+        "solves": nsol, "grid_shapes": len(set(c[0] for c in cases)),
+        "solves_per_s": nsol / t_batch, "batch_s": t_batch,
+        "sequential_s": t_seq, "sequential_solves_per_s": nsol / t_seq,
+        "speedup_vs_sequential": t_seq / t_batch,
+        "iterations_p50": float(np.percentile(steps, 50)),
+        "iterations_p99": float(np.percentile(steps, 99)),
+        "converged": int(sum(1 for o in out if o[1] == rism.STATUS_CONVERGED)),
+        "batch_equals_sequential": bool(all(a == b for a, b in zip(sfe_batch, sfe_seq))),
+        "host_threads": args.batch_threads,
+    }
+    if stepwise and budget.left() > 60:
+        # the same solves one after another with a host round trip per step
+        # (the reference's loop structure: rms read back before every update)
+        gs = make()
+        for g in gs:
+            g.kernel_timing(True)  # forces the step-by-step loop
+        sync()
+        t0 = time.perf_counter()
+        for g in gs:
+            g.run()
+        sync()
+        t_step = time.perf_counter() - t0
+        for g in gs:
+            g.close()
+        res["sequential_stepwise_s"] = t_step
+        res["speedup_vs_sequential_stepwise"] = t_step / t_batch
+    log(f"batch: {nsol / t_batch:.1f} solves/s, {t_seq / t_batch:.2f}x sequential")
+    return res
+
+
+# ---------------------------------------------------------------------------
+# HBM traffic from rocprofv3 counters (child processes)
+# ---------------------------------------------------------------------------
+COPY_BYTES = 2 << 30
+
+
+def kernel_key(name):
+    """Maps a dispatched kernel's name to the solver's kernel names."""
+    if "k_copy_stream" in name:
+        return "copy"
+    if "k_zfwd_closure" in name:
+        return "zfwd_closure"
+    if "k_xpass_oz" in name:
+        return "xpass_oz"
+    if "k_zinv_resid" in name:
+        return "zinv_resid"
+    if "k_an2_update" in name:
+        return "an2_update"
+    if "k_andm_update" in name:
+        return "an2_update"
+    if "k_ypass<" in name:
+        args = name.split("k_ypass<", 1)[1].split(">", 1)[0].split(",")
+        return "ypass_fwd" if args[2].strip() == "-1" else "ypass_inv"
+    return None
+
+
+def read_counters(outdir, counter):
+    """{kernel key: [value per dispatch]} from rocprofv3 csv output."""
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    vals = {}
+    for f in files:
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                k = kernel_key(row.get("Kernel_Name", ""))
+                if k:
+                    vals.setdefault(k, []).append(float(row["Counter_Value"]))
+    return vals
+
+
+def phase_traffic(args, budget, R, ctx):
+    """HBM bytes per launch of the hot-loop kernels from two rocprofv3 --pmc
+    child runs (FETCH_SIZE, then WRITE_SIZE), each scaled by what the counter
+    reads for a plain copy of COPY_BYTES in the same run."""
+    rp = shutil.which("rocprofv3")
+    if not rp:
+        return "rocprofv3 not found"
+    base = os.path.join(ROOT, "gpurun_out", "bench_traffic")
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        if budget.left() < 40:
+            return f"skipped: budget ({counter})"
+        outdir = os.path.join(base, counter.lower())
+        shutil.rmtree(outdir, ignore_errors=True)
+        cmd = ["timeout", "-s", "KILL", "120", rp, "--pmc", counter, "--output-format", "csv",
+               "-d", outdir, "-o", "trace", "--", sys.executable, os.path.abspath(__file__),
+               "--profile-run", args.workload, "--profile-steps", "8",
+               "--anderson-m", str(args.anderson_m)]
+        log("traffic:", " ".join(cmd[4:]))
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        vals = read_counters(outdir, counter)
+        if "copy" not in vals or not vals["copy"]:
+            return f"{counter}: no counter data (rc {p.returncode}): {p.stdout[-400:]}"
+        res[counter] = vals
+    fcal = COPY_BYTES / (np.mean(res["FETCH_SIZE"]["copy"]) * 1024.0)
+    wcal = COPY_BYTES / (np.mean(res["WRITE_SIZE"]["copy"]) * 1024.0)
+    table = {}
+    for k in res["FETCH_SIZE"]:
+        if k == "copy" or k not in res["WRITE_SIZE"]:
+            continue
+        f, w = res["FETCH_SIZE"][k], res["WRITE_SIZE"][k]
+        # the last launches: the mixing steps (the first two are Picard steps)
+        f, w = f[len(f) // 2:], w[len(w) // 2:]
+        rd = float(np.mean(f)) * 1024.0 * fcal
+        wr = float(np.mean(w)) * 1024.0 * wcal
+        table[k] = {"read_GB": rd / 1e9, "write_GB": wr / 1e9, "traffic_GB": (rd + wr) / 1e9}
+    return {"kernels": table, "fetch_factor": fcal, "write_factor": wcal,
+            "calibration": f"k_copy_stream of {COPY_BYTES} bytes in the same run"}
+
+
+def profile_run(args):
+    """Child of phase_traffic (and of the committed rocprofv3 summaries): a
+    plain copy of COPY_BYTES three times, then the set-up and a few
+    instrumented (step-by-step) steps of the workload."""
+    rism, grid, box, solv, sol, xvk, desc = workload(args.profile_run)
+    ms = rism.copy_bench(COPY_BYTES, 3)
+    print(f"copy {COPY_BYTES} B: {ms:.3f} ms ({2 * COPY_BYTES / ms / 1e6:.0f} GB/s)", flush=True)
+    prec = 1 if args.precision == "mixed" else 0
+    clos = closure_code(rism, args, "PSE3" if args.profile_run == "cfg3" else "KH")
+    g = rism.RISM3D(grid, box, solv, solu_dict(sol), clos, 2, args.beta, 0.2,
+                    anderson_m=args.anderson_m, precision=prec)
+    g.kernel_timing(True)
+    th, tm = g.run(maxstep=args.profile_steps, tol=1e-30)
+    ks = g.kernel_stats()
+    g.close()
+    print(json.dumps({"profile_run": args.profile_run, "steps": tm["steps"],
+                      "kernels_avg_ms": {k: v[1] / v[0] for k, v in ks.items() if v[0]}}),
+          flush=True)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline (the oracle on the host cores)
+# ---------------------------------------------------------------------------
+def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv):
     """FP64 oracle on the host cores, on a bounded sample extrapolated to one
     solve: one oracle step + one AN2 update at the full grid (synthetic
     fields: the cost does not depend on the values), plus LJ/Coulomb and fk
@@ -127,7 +604,6 @@ def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv, budget_s=30.0):
     s = O.read_xvk(solv_path)
     nx, ny, nz = grid
     N = nx * ny * nz
-    # potentials on `rows` y-rows of one z-plane, scaled by N / (rows * nx)
     rows = 16
     sub = (nx, rows, 1)
     xyz = np.ascontiguousarray(sol["xyz"], dtype=np.float64)
@@ -136,7 +612,6 @@ def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv, budget_s=30.0):
     t0 = time.time()
     O.potentials(sub, (box[0], box[1] * rows / ny, box[2] / nz), s, so)
     t_pot = (time.time() - t0) * N / (rows * nx)
-    # one step + one AN2 update at the full grid
     ga = np.linspace(0.01, 10.0, 64)
     chi = np.tile(np.eye(natv).reshape(-1), len(ga))
     indga = np.zeros(N, dtype=np.int32)
@@ -164,8 +639,41 @@ def cpu_baseline(grid, box, solv_path, sol, gpu_steps, natv, budget_s=30.0):
         "sample": (f"FP64 oracle (oracle/rism_oracle.c, OpenMP, own mixed-radix FFT): LJ+Coulomb"
                    f"+fk on {rows} rows x {nx} of one plane scaled x{N // (rows * nx)} = "
                    f"{t_pot:.1f} s; one step + one AN2 update at {nx}x{ny}x{nz} = "
-                   f"{t_step + t_an2:.1f} s, x {gpu_steps} steps (the GPU's count)"),
+                   f"{t_step + t_an2:.1f} s, x {gpu_steps} steps (the GPU's count); extrapolated"),
     }
+
+
+def cpu_full_cfg1():
+    """A full FP64 oracle solve of the 64^3 case, beside the GPU's time for the
+    same solve."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import gen_inputs as G
+    sol = centred(G.cfg1_solute())
+    s = O.read_xvk(os.path.join(GOLDEN, "tip3p_drism.xvk"))
+    so = O.Solute(np.asarray(sol["sig"], float), np.asarray(sol["eps"], float),
+                  np.asarray(sol["q"], float), np.ascontiguousarray(sol["xyz"], float))
+    t0 = time.time()
+    o = O.OracleRISM((64, 64, 64), (32.0, 32.0, 32.0), s, so, "KH", 2, 0.6, 0.2, 1e-6,
+                     10000).setup()
+    r = o.solve()
+    th = o.thermo()
+    return {"s_per_solve": time.time() - t0, "iterations": r.steps, "SFE_SC": th["SFE_SC"]}
+
+
+def gpu_cfg1(args):
+    rism, grid, box, solv, sol, xvk, desc = workload("cfg1")
+    g = rism.RISM3D(grid, box, solv, solu_dict(sol), rism.KH, 2, 0.6, 0.2)
+    g.run()
+    sync()
+    t0 = time.perf_counter()
+    n = 5
+    for _ in range(n):
+        th, tm = g.run()
+    sync()
+    wall = (time.perf_counter() - t0) / n
+    g.close()
+    return {"s_per_solve": wall, "iterations": tm["steps"], "SFE_SC": th["SFE_SC"]}
 
 
 def closure_code(rism, args, default):
@@ -173,16 +681,15 @@ def closure_code(rism, args, default):
     return {"KH": rism.KH, "HNC": rism.HNC, "PSE3": rism.PSE(3)}[name]
 
 
+# ---------------------------------------------------------------------------
+# other workloads (not the headline line)
+# ---------------------------------------------------------------------------
 def bench_medium(args):
     """Batched medium-size solutes in mixed precision: 32 protein-like
     clusters (500-4000 atoms log-uniform, 0.1 atoms/A^3, net charge 0, seeds
     1005000 + i) in 0.2 M NaCl TIP3P (tip3p_nacl02_drism.xvk), boxes of
     extent + 12 A per side at 0.5 A spacing with per-axis sizes from
-    {128, 144, 160, 180, 192}; HNC, AN2 2 0.6 0.2, tol 1e-6.  Times the
-    mixed-precision batch (rism_run_batch) against the same mixed solves one
-    after another, and reports |dSFE| of every solute against its FP64 solve
-    (run as a sync-free FP64 batch, also timed)."""
-    import torch
+    {128, 144, 160, 180, 192}; the configured HNC closure by default."""
     import gen_inputs as G
     import __graft_entry__ as ge
     rism = ge.load_rism3d()
@@ -194,13 +701,9 @@ def bench_medium(args):
     for i in range(nsol):
         sol = G.medium_solute(i)
         grid, box = G.batch_grid(sol["xyz"], sizes=G.MEDIUM_SIZES)
-        xyz = sol["xyz"]
-        mn, mx = xyz.min(0), xyz.max(0)
-        shift = np.array([np.floor(-(mx[k] - mn[k]) / 2 - mn[k] + 0.5) for k in range(3)])
-        cases.append((grid, box, {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"],
-                                  "uxyz": xyz + shift}))
+        cases.append((grid, box, solu_dict(centred(sol))))
         natoms.append(len(sol["sig"]))
-    log(f"[bench] batch-medium: {nsol} solutes ({min(natoms)}-{max(natoms)} atoms), "
+    log(f"batch-medium: {nsol} solutes ({min(natoms)}-{max(natoms)} atoms), "
         f"{len(set(c[0] for c in cases))} grid shapes")
 
     def make(prec):
@@ -209,383 +712,209 @@ def bench_medium(args):
 
     def timed_batch(prec):
         gs = make(prec)
-        torch.cuda.synchronize()
+        sync()
         t0 = time.perf_counter()
         out = rism.run_batch(gs, threads=args.batch_threads)
-        torch.cuda.synchronize()
+        sync()
         t = time.perf_counter() - t0
         for g in gs:
             g.close()
         return t, out
 
-    w = make(1)[:2]  # warm-up (module load, first launches)
+    w = make(1)[:2]
     rism.run_batch(w, threads=args.batch_threads)
     for g in w:
         g.close()
     t_mixed, out_m = timed_batch(1)
-    log(f"[bench] mixed batch: {t_mixed:.2f} s")
     gs = make(1)
-    torch.cuda.synchronize()
+    sync()
     t0 = time.perf_counter()
     seq = [g.run()[0]["SFE_SC"] for g in gs]
-    torch.cuda.synchronize()
+    sync()
     t_seq = time.perf_counter() - t0
     for g in gs:
         g.close()
-    log(f"[bench] mixed one after another: {t_seq:.2f} s")
     t_fp64, out_d = timed_batch(0)
-    log(f"[bench] FP64 batch: {t_fp64:.2f} s")
     conv = [i for i in range(nsol) if out_m[i][1] == rism.STATUS_CONVERGED
             and out_d[i][1] == rism.STATUS_CONVERGED]
-    dmu = np.array([abs(out_m[i][0]["SFE_SC"] - out_d[i][0]["SFE_SC"]) / 4184.0 for i in conv])
+    dmu = np.array([abs(out_m[i][0]["SFE_SC"] - out_d[i][0]["SFE_SC"]) / KCAL for i in conv])
     steps = np.array([o[2] for o in out_m])
-    res = {
-        "metric": METRIC,
-        "value": nsol / t_mixed,
-        "unit": "solves/s",
-        "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": 1000.0 * t_mixed,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+    res = base_line(args)
+    res.update({
+        "value": nsol / t_mixed, "unit": "solves/s", "steps": 1, "warmup": 1,
+        "ms_per_step": 1000.0 * t_mixed, "higher_is_better": True,
         "dtype": "f32 bulk, f64 sums and FP64 steps",
-        "data": "synthetic",
-        "config": {
-            "workload": ("batch-medium: %d protein-like clusters (500-4000 atoms, seeds "
-                         "1005000+i, 0 e) in 0.2 M NaCl TIP3P (chi from the project's 1D "
-                         "DRISM-KH, eps 78.4), boxes = extent + 12 A/side at 0.5 A, grids "
-                         "from {128..192}^3, %s, AN2 2 0.6 0.2, tol 1e-6, mixed precision"
-                         % (nsol, cname)),
-            "solves": nsol, "grid_shapes": len(set(c[0] for c in cases)),
-            "atoms_min_max": [min(natoms), max(natoms)],
-            "batch_s": t_mixed, "sequential_s": t_seq,
-            "speedup_vs_sequential": t_seq / t_mixed,
-            "batch_equals_sequential": bool(all(out_m[i][0]["SFE_SC"] == seq[i]
-                                                for i in range(nsol))),
-            "fp64_batch_s": t_fp64, "speedup_vs_fp64_batch": t_fp64 / t_mixed,
-            "iterations_p50": float(np.percentile(steps, 50)),
-            "iterations_p99": float(np.percentile(steps, 99)),
-            "converged_mixed": int(sum(1 for o in out_m if o[1] == rism.STATUS_CONVERGED)),
-            "converged_fp64": int(sum(1 for o in out_d if o[1] == rism.STATUS_CONVERGED)),
-            "dSFE_kcal_per_mol_max": float(dmu.max()) if len(dmu) else None,
-            "dSFE_kcal_per_mol_p50": float(np.percentile(dmu, 50)) if len(dmu) else None,
-            "within_0.05_kcal_per_mol": int((dmu <= 0.05).sum()),
-            "host_threads": args.batch_threads,
-        },
+    })
+    res["config"] = {
+        "workload": ("batch-medium: %d protein-like clusters (500-4000 atoms, seeds 1005000+i, "
+                     "0 e) in 0.2 M NaCl TIP3P (chi from the project's 1D DRISM-KH, eps 78.4), "
+                     "boxes = extent + 12 A/side at 0.5 A, grids from {128..192}^3, %s, AN2 "
+                     "2 0.6 0.2, tol 1e-6, mixed precision" % (nsol, cname)),
+        "solves": nsol, "grid_shapes": len(set(c[0] for c in cases)),
+        "atoms_min_max": [min(natoms), max(natoms)],
+        "batch_s": t_mixed, "sequential_s": t_seq, "speedup_vs_sequential": t_seq / t_mixed,
+        "batch_equals_sequential": bool(all(out_m[i][0]["SFE_SC"] == seq[i]
+                                            for i in range(nsol))),
+        "fp64_batch_s": t_fp64, "speedup_vs_fp64_batch": t_fp64 / t_mixed,
+        "iterations_p50": float(np.percentile(steps, 50)),
+        "iterations_p99": float(np.percentile(steps, 99)),
+        "converged_mixed": int(sum(1 for o in out_m if o[1] == rism.STATUS_CONVERGED)),
+        "converged_fp64": int(sum(1 for o in out_d if o[1] == rism.STATUS_CONVERGED)),
+        "dSFE_kcal_per_mol_max": float(dmu.max()) if len(dmu) else None,
+        "dSFE_kcal_per_mol_p50": float(np.percentile(dmu, 50)) if len(dmu) else None,
+        "within_0.05_kcal_per_mol": int((dmu <= 0.05).sum()),
     }
     print(json.dumps(res), flush=True)
 
 
-def bench_batch(args):
-    """Batched small solutes: 256 random-walk solutes of 8-60 atoms (seeds
-    1004000 + i, net charge -1/0/+1) in TIP3P water (tip3p_drism.xvk), each in
-    its own box of extent + 12 A padding per side at 0.5 A spacing, per-axis
-    grid sizes from {48, 60, 64, 72, 80, 96}; KH, AN2 2 0.6 0.2, tol 1e-6,
-    FP64.  Times the batch (rism_run_batch, one stream per solute, host
-    threads enqueueing) against the same solves run one after another."""
-    import torch
-    import gen_inputs as G
-    import __graft_entry__ as ge
-    rism = ge.load_rism3d()
-    solv = rism.read_xvk(os.path.join(GOLDEN, "tip3p_drism.xvk"))
-    nsol = args.batch_size if args.batch_size else 256
-    cases = []
-    for i in range(nsol):
-        sol = G.batch_solute(i)
-        grid, box = G.batch_grid(sol["xyz"])
-        xyz = sol["xyz"]
-        mn, mx = xyz.min(0), xyz.max(0)
-        shift = np.array([np.floor(-(mx[k] - mn[k]) / 2 - mn[k] + 0.5) for k in range(3)])
-        cases.append((grid, box, {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"],
-                                  "uxyz": xyz + shift}))
-    log(f"[bench] batch: {nsol} solutes, {len(set(c[0] for c in cases))} grid shapes")
-
-    prec = 1 if args.precision == "mixed" else 0
-
-    def make():
-        return [rism.RISM3D(g, b, solv, su, rism.KH, 2, 0.6, 0.2, precision=prec)
-                for g, b, su in cases]
-
-    # warm-up (module load, first launches)
-    w = make()[:4]
-    rism.run_batch(w, threads=args.batch_threads)
-    for g in w:
-        g.close()
-    # batched
-    gs = make()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out = rism.run_batch(gs, threads=args.batch_threads)
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
-    steps = np.array([o[2] for o in out])
-    status = [o[1] for o in out]
-    sfe_batch = [o[0]["SFE_SC"] for o in out]
-    for g in gs:
-        g.close()
-    # the same solves one after another
-    gs = make()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sfe_seq = []
-    for g in gs:
-        th, tm = g.run()
-        sfe_seq.append(th["SFE_SC"])
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
-    for g in gs:
-        g.close()
-    same = all(a == b for a, b in zip(sfe_batch, sfe_seq))
-    # the same solves one after another with a host round trip per step (the
-    # reference's loop structure: rms read back before every update)
-    gs = make()
-    for g in gs:
-        g.kernel_timing(True)  # per-kernel events: forces the step-by-step loop
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for g in gs:
-        g.run()
-    torch.cuda.synchronize()
-    t_step = time.perf_counter() - t0
-    for g in gs:
-        g.close()
-    res = {
-        "metric": METRIC,
-        "value": nsol / t_batch,
-        "unit": "solves/s",
-        "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": 1000.0 * t_batch,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 bulk, f64 sums and FP64 steps" if prec else "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": ("batch: %d random-walk solutes (8-60 atoms, seeds 1004000+i, net "
-                         "charge -1/0/+1) in TIP3P water (chi from the project's 1D DRISM-KH, "
-                         "eps 78.4), boxes = extent + 12 A/side at 0.5 A, grids from "
-                         "{48..96}^3, KH, AN2 2 0.6 0.2, tol 1e-6, %s"
-                         % (nsol, "mixed precision" if prec else "FP64")),
-            "solves": nsol, "grid_shapes": len(set(c[0] for c in cases)),
-            "batch_s": t_batch, "sequential_s": t_seq,
-            "sequential_solves_per_s": nsol / t_seq,
-            "speedup_vs_sequential": t_seq / t_batch,
-            "sequential_stepwise_s": t_step,
-            "speedup_vs_sequential_stepwise": t_step / t_batch,
-            "iterations_p50": float(np.percentile(steps, 50)),
-            "iterations_p99": float(np.percentile(steps, 99)),
-            "converged": int(sum(1 for s in status if s == rism.STATUS_CONVERGED)),
-            "batch_equals_sequential": bool(same),
-            "host_threads": args.batch_threads,
-        },
+def base_line(args):
+    return {
+        "metric": METRIC, "value": None, "unit": "s/solve", "n_gpus": 1, "steps": 0,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "budget_s": args.budget},
+        "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(res), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# main
+# ---------------------------------------------------------------------------
+_print_lock = threading.Lock()
+_printed = [False]
+
+
+def emit(obj):
+    with _print_lock:
+        if _printed[0]:
+            return
+        _printed[0] = True
+        print(json.dumps(obj), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed solves (at most; the budget may stop earlier)")
+    ap.add_argument("--warmup", type=int, default=1,
+                    help="untimed warm-up solves (at most; at least one runs)")
+    ap.add_argument("--budget", type=float, default=450.0,
+                    help="wall-clock seconds from process start for the whole run")
     ap.add_argument("--workload", default="cfg2",
-                    choices=["cfg2", "cfg1", "cfg3", "mobley", "batch", "batch-medium"],
-                    help="cfg2 (512^3, default), cfg1 (64^3), cfg3 (256^3, PSE-3), mobley, "
-                         "batch (256 small solutes) or batch-medium (32 medium solutes, "
-                         "mixed precision)")
+                    choices=["cfg2", "cfg1", "cfg3", "mobley", "test2", "batch", "batch-medium"])
     ap.add_argument("--closure", default=None, choices=["KH", "HNC", "PSE3"],
                     help="override the workload's closure (cfg3: PSE3, batch-medium: HNC, "
                          "others: KH)")
-    ap.add_argument("--batch-size", type=int, default=None,
-                    help="solutes of a batch workload (256 small / 32 medium)")
+    ap.add_argument("--batch-size", type=int, default=None)
     ap.add_argument("--batch-threads", type=int, default=8)
-    ap.add_argument("--anderson-m", type=int, default=None,
-                    help="history length of Anderson(m) mixing (0: the reference's AN2)")
-    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
-                    help="fp64: the reference's precision; mixed: FP32 bulk + FP64 sums and "
-                         "FP64 steps; both: time both (value = FP64); the small-solute batch "
-                         "runs mixed contexts with mixed, FP64 ones otherwise")
+    ap.add_argument("--anderson-m", type=int, default=0,
+                    help="history length of Anderson(m) mixing for the headline (0: AN2)")
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--beta", type=float, default=0.6,
                     help="mixing coefficient (AN2's m; beta of Anderson(m))")
+    ap.add_argument("--only", default=None,
+                    help="comma list of extra phases to run (mixed,batch,test2,traffic,cpu,"
+                         "anderson10); default all, as the budget allows")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--maxstep", type=int, default=10000,
-                    help="iteration cap (profiling runs only; the metric needs convergence)")
-    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
-                    help="only run a plain device copy of GIB GiB five times (scales the "
-                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
+    ap.add_argument("--maxstep", type=int, default=10000)
+    ap.add_argument("--profile-steps", type=int, default=40,
+                    help="steps of the instrumented per-kernel run")
+    ap.add_argument("--warm-cap", type=float, default=20.0)
+    ap.add_argument("--timed-cap", type=float, default=90.0)
+    ap.add_argument("--mixed-cap", type=float, default=45.0)
+    ap.add_argument("--profile-run", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs one solve per GPU and has no "
               "multi-GPU path; run with --gpus 1", flush=True)
         sys.exit(2)
-
-    import torch
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    torch.cuda.set_device(0)
-    if args.copy_calibration > 0:
-        n = int(args.copy_calibration * (1 << 30)) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 5}), flush=True)
-        return
-
-    if args.workload == "batch":
-        bench_batch(args)
+    if device_count() < 1:
+        raise SystemExit("bench.py needs a GPU")
+    if hip().hipSetDevice(0) != 0:
+        raise SystemExit("hipSetDevice(0) failed")
+    if args.profile_run:
+        profile_run(args)
         return
     if args.workload == "batch-medium":
         bench_medium(args)
         return
-    rism, grid, box, solv, sol, xvk, desc = workload(args.workload)
-    solu = {"usig": sol["sig"], "ueps": sol["eps"], "uq": sol["q"], "uxyz": sol["xyz"]}
-    log(f"[bench] workload {args.workload}: grid {grid}, natu {len(sol['sig'])}, "
-        f"natv {solv['natv']}")
-    am = args.anderson_m if args.anderson_m is not None else 0
-    clos = closure_code(rism, args, "PSE3" if args.workload == "cfg3" else "KH")
-    g = rism.RISM3D(grid, box, solv, solu, clos, 2, args.beta, 0.2, anderson_m=am)
-    for i in range(args.warmup):
-        th, tm = g.run(maxstep=args.maxstep, tol=1e-6)
-        log(f"[bench] warmup {i}: {tm}")
-    g.kernel_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    results = []
-    for i in range(args.steps):
-        th, tm = g.run(maxstep=args.maxstep, tol=1e-6)
-        results.append((th, tm))
-        log(f"[bench] step {i}: {tm} SFE_SC {th['SFE_SC']:.5f}")
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ks = g.kernel_stats()
-    g.kernel_timing(False)
-    N = grid[0] * grid[1] * grid[2]
-    # h of the FP64 solve for the rms(dh) of the mixed one (4.3 GB of host
-    # memory at 512^3 and 4 sites)
-    h64 = g.fields(rism.FIELD_H) if args.precision in ("mixed", "both") else None
-    g.close()
-    th, tm = results[-1]
-    if tm["status"] != rism.STATUS_CONVERGED:
-        log(f"[bench] WARNING: solve did not converge (status {tm['status']})")
-    ms_per_step = 1000.0 * wall / args.steps
-    natv = solv["natv"]
-    iters = tm["steps"]
 
-    # roofline of the dominant hot-loop kernel
-    ab = algorithmic_bytes(natv, N, am if am > 0 else 2)
-    hot = {k: v for k, v in ks.items() if k in ab and v[0] > 0}
-    dom = max(hot, key=lambda k: hot[k][1])
-    launches, total_ms = hot[dom]
-    avg_s = total_ms / launches / 1000.0
-    achieved = ab[dom] / avg_s / 1e9
-    per_kernel = {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1),
-                      "GB/s": (ab[k] / (v[1] / v[0] / 1000.0) / 1e9) if k in ab and v[0] else None}
-                  for k, v in ks.items() if v[0] > 0}
-    iter_bytes = sum(ab.values())
-    iter_ms = tm["iterate_ms"] / max(iters, 1)
-    # the fused lower bound of one iteration, each 3D transform counted as
-    # one HBM round trip: N [(12 + 3 m) n + 2] w bytes, m history pairs (2
-    # for AN2), w bytes per bulk value
-    mh = am if am > 0 else 2
+    budget = Budget(args.budget)
+    R = {"out": base_line(args)}
+    # a last-resort line if the run overstays its budget (a slow box): the
+    # phases measured so far
+    hard = max(30.0, min(args.budget + 90.0, 570.0) - (time.time() - T_START))
 
-    def fused_bytes(w):
-        return N * ((12 + 3 * mh) * natv + 2) * w
+    def watchdog():
+        R["out"]["truncated"] = "budget exceeded; later phases not run"
+        emit(R["out"])
+        os._exit(0)
 
-    log("[bench] per-kernel:", json.dumps(per_kernel, indent=None))
+    wd = threading.Timer(hard, watchdog)
+    wd.daemon = True
+    wd.start()
 
-    mixed = None
-    if args.precision in ("mixed", "both"):
-        gm = rism.RISM3D(grid, box, solv, solu, clos, 2, args.beta, 0.2, anderson_m=am,
-                         precision=1)
-        for i in range(args.warmup):
-            thm, tmm = gm.run(maxstep=args.maxstep, tol=1e-6)
-            log(f"[bench] mixed warmup {i}: {tmm}")
-        gm.kernel_timing(True)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for i in range(args.steps):
-            thm, tmm = gm.run(maxstep=args.maxstep, tol=1e-6)
-            log(f"[bench] mixed step {i}: {tmm} SFE_SC {thm['SFE_SC']:.5f}")
-        torch.cuda.synchronize()
-        wall_m = time.perf_counter() - t1
-        ksm = gm.kernel_stats()
-        gm.kernel_timing(False)
-        rms_dh = None
-        if h64 is not None:
-            h32 = gm.fields(rism.FIELD_H)
-            h32 -= h64
-            rms_dh = float(np.sqrt(np.mean(h32 * h32)))
-            del h32
-        gm.close()
-        dmu = thm["SFE_SC"] - th["SFE_SC"]
-        mixed = {
-            "s_per_solve": wall_m / args.steps,
-            "speedup_vs_fp64": (ms_per_step / 1000.0) / (wall_m / args.steps),
-            "iterations": tmm["steps"], "iterations_fp32": tmm["steps_fp32"],
-            "status": tmm["status"], "setup_ms": tmm["setup_ms"],
-            "iterate_ms": tmm["iterate_ms"],
-            "SFE_SC": thm["SFE_SC"], "dSFE_SC_J_per_mol": dmu,
-            "dSFE_SC_kcal_per_mol": dmu / 4184.0,
-            "gate_0.05_kcal_per_mol": abs(dmu) <= 0.05 * 4184.0,
-            "dPMV": thm["PMV"] - th["PMV"],
-            "rms_dh": rms_dh,
-            # FP32 steps at w = 4, FP64 steps at w = 8, over the iteration phase
-            "iteration_fused_bound_frac": ((tmm["steps_fp32"] * fused_bytes(4)
-                                            + (tmm["steps"] - tmm["steps_fp32"]) * fused_bytes(8))
-                                           / (tmm["iterate_ms"] / 1000.0) / (HBM_PEAK_GBS * 1e9)),
-            "kernels_avg_ms": {k: v[1] / max(v[0], 1) for k, v in ksm.items() if v[0] > 0},
-        }
-        log(f"[bench] mixed: {mixed}")
+    if args.workload == "batch":
+        res = phase_batch(args, budget, R, nsol=args.batch_size or 256,
+                          prec=1 if args.precision == "mixed" else 0)
+        R["out"].update({"value": res["solves_per_s"], "unit": "solves/s", "steps": 1,
+                         "warmup": 1, "ms_per_step": 1000.0 * res["batch_s"],
+                         "higher_is_better": True})
+        R["out"]["config"].update(res)
+        emit(R["out"])
+        return
 
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] CPU baseline (oracle) ...")
-        cpu = cpu_baseline(grid, box, os.path.join(GOLDEN, xvk), sol, iters, natv)
-        log(f"[bench] CPU baseline: {cpu}")
-
-    out = {
-        "metric": METRIC,
-        "value": ms_per_step / 1000.0,
-        "unit": "s/solve",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": desc,
-            "grid": list(grid), "box": list(box), "natu": int(len(sol["sig"])),
-            "natv": int(natv), "closure": args.closure or ("PSE3" if args.workload == "cfg3"
-                                                            else "KH"),
-            "anderson_m": am, "iterations": iters, "status": tm["status"],
-            "setup_ms": tm["setup_ms"], "iterate_ms": tm["iterate_ms"],
-            "thermo_ms": tm["thermo_ms"], "ms_per_iteration": iter_ms,
-            "iteration_GBps_algorithmic": iter_bytes / (iter_ms / 1000.0) / 1e9,
-            "iteration_fused_bound_GB": fused_bytes(8) / 1e9,
-            "iteration_fused_bound_frac": fused_bytes(8) / (iter_ms / 1000.0) / (HBM_PEAK_GBS * 1e9),
-            "SFE_SC": th["SFE_SC"], "SFE_GF": th["SFE_GF"], "PMV": th["PMV"],
-            "kernels": per_kernel,
-        },
-        "roofline": {
-            "kernel": dom,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (MEASURED_TRAFFIC_CFG2.get(dom)
-                        if args.workload == "cfg2" and am == 0 else None),
-            "algorithmic_bytes_per_launch": ab[dom],
-            "avg_launch_ms": avg_s * 1000.0,
-        },
-        "cpu_baseline": cpu,
-        "mixed": mixed,
-    }
-    print(json.dumps(out), flush=True)
+    want = set((args.only or "mixed,batch,test2,traffic,cpu,anderson10").split(","))
+    ctx = phase_headline(args, budget, R)
+    log(f"headline done, {budget.left():.0f} s of budget left")
+    phases = []
+    if args.precision in ("mixed", "both") and "mixed" in want:
+        phases.append(("mixed", 30 + 1.6 * R["out"]["value"],
+                       lambda: phase_mixed(args, budget, R, ctx)))
+    if args.workload == "cfg2" and "batch" in want:
+        phases.append(("batch", 40, lambda: R["out"].__setitem__(
+            "batch", phase_batch(args, budget, R))))
+    if args.workload == "cfg2" and "test2" in want:
+        phases.append(("test2", 50, lambda: phase_test2(args, budget, R)))
+    if "traffic" in want:
+        def run_traffic():
+            tr = phase_traffic(args, budget, R, ctx)
+            R["out"]["hbm_traffic"] = tr
+            if isinstance(tr, dict) and ctx["dom"] in tr["kernels"]:
+                R["out"]["roofline"]["traffic"] = tr["kernels"][ctx["dom"]]["traffic_GB"] * 1e9
+        phases.append(("traffic", 45, run_traffic))
+    if not args.no_cpu and "cpu" in want:
+        def run_cpu():
+            cb = cpu_baseline(ctx["grid"], ctx["box"], os.path.join(GOLDEN, ctx["xvk"]),
+                              ctx["sol"], ctx["iters"], ctx["natv"])
+            if budget.left() > 60:
+                c1 = cpu_full_cfg1()
+                g1 = gpu_cfg1(args)
+                cb["full_solve_64^3"] = {"cpu": c1, "gpu": g1,
+                                         "gpu_speedup": c1["s_per_solve"] / g1["s_per_solve"]}
+            R["out"]["cpu_baseline"] = cb
+        phases.append(("cpu", 40, run_cpu))
+    if args.workload == "cfg2" and "anderson10" in want:
+        phases.append(("anderson10", 60, lambda: phase_anderson(args, budget, R, ctx)))
+    skipped = []
+    for name, need, fn in phases:
+        if budget.left() < need:
+            skipped.append(name)
+            log(f"skip {name}: {budget.left():.0f} s left, needs ~{need:.0f} s")
+            continue
+        t0 = time.time()
+        try:
+            fn()
+        except Exception as e:  # a failed side phase must not lose the headline
+            R["out"].setdefault("errors", {})[name] = repr(e)[:400]
+            log(f"phase {name} failed: {e!r}")
+        log(f"phase {name}: {time.time() - t0:.1f} s, {budget.left():.0f} s left")
+    if skipped:
+        R["out"]["skipped_phases"] = skipped
+    R["out"]["config"]["wall_s"] = time.time() - T_START
+    wd.cancel()
+    emit(R["out"])
 
 
 if __name__ == "__main__":

@@ -328,6 +328,14 @@ int rism_k_get_fk(rism_ctx *ctx, double *out);
  * count (remaining warm-up steps), binary (next slot). */
 int rism_k_an2_state(rism_ctx *ctx, double *x, double *s, int *count, int *binary);
 
+/* ---- diagnostics (no counterpart in the reference) ---------------------- */
+/* Copies `bytes` (rounded down to whole 16-byte vectors) between two device
+ * buffers of its own `reps` times with a plain streaming kernel and returns
+ * the average time per copy: the reference point for HBM traffic counters
+ * (each copy reads and writes `bytes`) and for the achievable copy
+ * bandwidth.  Allocates and frees its buffers. */
+int rism_k_copy_bench(size_t bytes, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

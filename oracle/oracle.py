@@ -84,6 +84,13 @@ def lib():
     L.or_fk.argtypes = [c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_int, _dp, _dp, c_dbl,
                         _dp]
     L.or_fk.restype = None
+    rows = [c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_int, c_int, c_int, c_int]
+    L.or_lj_rows.argtypes = rows + [c_int, _dp, _dp, _dp, c_int, _dp, _dp, c_dbl, c_dbl, _dp]
+    L.or_lj_rows.restype = None
+    L.or_coulomb_rows.argtypes = rows + [c_int, _dp, _dp, c_dbl, _dp, _dp]
+    L.or_coulomb_rows.restype = None
+    L.or_fk_rows.argtypes = rows + [c_int, _dp, _dp, c_dbl, _dp]
+    L.or_fk_rows.restype = None
     L.or_closure.argtypes = [c_int, c_sz, _dp, _dp, _dp]
     L.or_closure.restype = None
     L.or_calculate.argtypes = [c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_int, c_int, _dp,
@@ -307,6 +314,58 @@ def potentials(grid, box, solv: Solvent, sol: Solute, lam=1.0, adswitch=0):
     u = u.reshape(solv.natv, N)
     u += solv.q[:, None] * de[None, :]
     return u.reshape(-1).copy(), fr, fk
+
+
+def potentials_rows(grid, box, solv: Solvent, sol: Solute, z0, nzs, y0, nys):
+    """potentials() on the rows z in [z0, z0 + nzs), y in [y0, y0 + nys) of the
+    full grid (every x): u [natv][nzs][nys][nx], fr [nzs][nys][nx], and fk on
+    the same index block of the full k-grid, complex."""
+    L = lib()
+    nx, ny, nz = (int(g) for g in grid)
+    b = [float(x) for x in box]
+    blk = [nx, ny, nz] + b + [int(z0), int(nzs), int(y0), int(nys)]
+    n = nx * int(nzs) * int(nys)
+    xyz = np.ascontiguousarray(sol.xyz.reshape(-1), dtype=np.float64)
+    q = np.ascontiguousarray(sol.q, dtype=np.float64)
+    u = np.zeros(solv.natv * n)
+    L.or_lj_rows(*blk, sol.natu, xyz, np.ascontiguousarray(sol.sig),
+                 np.ascontiguousarray(sol.eps), solv.natv, np.ascontiguousarray(solv.sig),
+                 np.ascontiguousarray(solv.eps), solv.temper, 1.0, u)
+    de, fr, fk = np.zeros(n), np.zeros(n), np.zeros(2 * n)
+    ub = L.or_ubeta(solv.temper, 1.0)
+    L.or_coulomb_rows(*blk, sol.natu, xyz, q, ub, de, fr)
+    L.or_fk_rows(*blk, sol.natu, xyz, q, ub, fk)
+    u = u.reshape(solv.natv, n) + solv.q[:, None] * de[None, :]
+    shp = (int(nzs), int(nys), nx)
+    return (u.reshape((solv.natv,) + shp), fr.reshape(shp),
+            fk.view(np.complex128).reshape(shp))
+
+
+def cubic_shells(grid, box):
+    """Shells of a cubic box in closed form: with odd o = 2 j - N + 1 per axis,
+    |k|^2 = (pi/L)^2 (ox^2 + oy^2 + oz^2) = (pi/L)^2 (8 i + 3), so the shell of
+    a point is i = (ox^2 + oy^2 + oz^2 - 3) / 8 and ga[i] = (pi/L) sqrt(8 i + 3)
+    for every i up to the largest.  Equals kshells() up to the labelling of
+    shells (and unrealised i), checked in test_oracle_setup.py; used where
+    sorting N values would be slow (512^3)."""
+    nx, ny, nz = (int(g) for g in grid)
+    assert nx == ny == nz and box[0] == box[1] == box[2]
+    o = (2 * np.arange(nx, dtype=np.int64) - nx + 1) ** 2
+    s = (o[:, None, None] + o[None, :, None] + o[None, None, :]).reshape(-1)
+    indga = ((s - 3) // 8).astype(np.int32)
+    imax = int(indga.max())
+    ga = (np.pi / float(box[0])) * np.sqrt(8.0 * np.arange(imax + 1) + 3.0)
+    return ga, indga
+
+
+def full_spectrum(half):
+    """The full k-grid spectrum [nz][ny][nx] of a real field from its k_z > 0
+    half [nz/2][ny][nx]: F(-k) = conj F(k) and k_{N-1-j} = -k_j per axis."""
+    m, ny, nx = half.shape
+    out = np.empty((2 * m, ny, nx), dtype=np.complex128)
+    out[m:] = half
+    out[:m] = np.conj(half[::-1, ::-1, ::-1])
+    return out
 
 
 @dataclass

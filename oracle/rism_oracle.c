@@ -47,15 +47,19 @@ static inline cplx cmul(cplx a, cplx b) {
 }
 
 /* ------------------------------------------------------------------------- */
-/* 1D complex FFT of arbitrary length: recursive mixed radix (Cooley-Tukey)  */
-/* with a direct DFT for prime factors.  Used line by line by or_fft3d.      */
+/* 1D complex DFT of any length, VB lines at a time: an iterative mixed-radix */
+/* self-sorting (Stockham) transform on planar blocks [i][re|im][VB lane].    */
+/* Radix 4 and 2 have unrolled butterflies; radix 3, 5 and any other prime  */
+/* are evaluated from the DFT definition.  Used by or_fft3d.                 */
 /* ------------------------------------------------------------------------- */
+#define OR_VB 8
 
 typedef struct {
   int n;
   int nf;
   int fac[32];
-  cplx *w; /* w[k] = exp(-2 pi i k / n) */
+  cplx *tw[32]; /* stage s: tw[s][k * (r - 1) + q - 1] = e^{-2 pi i q k / (Ns r)}, k < Ns */
+  cplx *wr[32]; /* stage s: wr[s][t] = e^{-2 pi i t / r}, t < r                          */
 } fftplan;
 
 static void plan_init(fftplan *p, int n) {
@@ -68,59 +72,134 @@ static void plan_init(fftplan *p, int n) {
   for (int f = 3; m > 1; f += 2) {
     while (m % f == 0) { p->fac[p->nf++] = f; m /= f; }
   }
-  p->w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
-  for (int k = 0; k < n; ++k) {
-    double a = -2.0 * M_PI * (double)k / (double)n;
-    p->w[k].re = cos(a);
-    p->w[k].im = sin(a);
+  int ns = 1;
+  for (int st = 0; st < p->nf; ++st) {
+    int r = p->fac[st];
+    p->tw[st] = (cplx *)malloc(sizeof(cplx) * (size_t)ns * (r - 1) + sizeof(cplx));
+    for (int k = 0; k < ns; ++k)
+      for (int q = 1; q < r; ++q) {
+        double a = -2.0 * M_PI * (double)q * (double)k / ((double)ns * r);
+        p->tw[st][k * (r - 1) + q - 1].re = cos(a);
+        p->tw[st][k * (r - 1) + q - 1].im = sin(a);
+      }
+    p->wr[st] = (cplx *)malloc(sizeof(cplx) * (size_t)r);
+    for (int t = 0; t < r; ++t) {
+      double a = -2.0 * M_PI * (double)t / (double)r;
+      p->wr[st][t].re = cos(a);
+      p->wr[st][t].im = sin(a);
+    }
+    ns *= r;
   }
 }
 
-static void plan_free(fftplan *p) { free(p->w); }
+static void plan_free(fftplan *p) {
+  for (int st = 0; st < p->nf; ++st) {
+    free(p->tw[st]);
+    free(p->wr[st]);
+  }
+}
 
-/* out[k] = sum_j in[j*stride] * exp(sign * 2 pi i j k / n), recursive DIT. */
-static void fft_rec(const fftplan *p, int lev, int n, const cplx *in, int stride,
-                    cplx *out, int sign, cplx *scratch) {
-  if (n == 1) { out[0] = in[0]; return; }
-  int r = p->fac[lev];
-  int m = n / r;
-  int tstep = p->n / n; /* twiddle stride into the length-p->n table */
-  /* sub-transforms of the r decimated sequences */
-  for (int q = 0; q < r; ++q)
-    fft_rec(p, lev + 1, m, in + (size_t)q * stride, stride * r, out + (size_t)q * m,
-            sign, scratch + n);
-  /* butterflies */
-  cplx *t = scratch;
-  for (int k = 0; k < m; ++k) {
+/* element i of a planar block: re at blk + i*2*OR_VB, im OR_VB further */
+#define RE(blk, i) ((blk) + (size_t)(i) * 2 * OR_VB)
+#define IM(blk, i) ((blk) + (size_t)(i) * 2 * OR_VB + OR_VB)
+
+/* One Stockham stage of radix r over the block: butterfly j (j < n/r, k = j mod
+ * Ns) reads elements j + q n/r, multiplies element q by e^{sign 2 pi i q k /
+ * (Ns r)}, takes the length-r DFT and writes output t to (j/Ns) Ns r + k + t Ns.
+ * sign = -1 forward, +1 inverse (unnormalised). */
+static void stage(const fftplan *p, int st, int ns, const double *in, double *out, int sign) {
+  const int n = p->n, r = p->fac[st], m = n / r;
+  const cplx *tw = p->tw[st], *wr = p->wr[st];
+  double vr[8][OR_VB], vi[8][OR_VB];
+  double *gr = NULL, *gi = NULL;
+  if (r > 8) {
+    gr = (double *)malloc(sizeof(double) * (size_t)r * OR_VB);
+    gi = (double *)malloc(sizeof(double) * (size_t)r * OR_VB);
+  }
+  for (int j = 0; j < m; ++j) {
+    const int k = j % ns;
+    const int o0 = (j / ns) * ns * r + k;
+    double *xr = (r > 8) ? gr : &vr[0][0];
+    double *xi = (r > 8) ? gi : &vi[0][0];
     for (int q = 0; q < r; ++q) {
-      cplx v = out[q * m + k];
+      const double *sr = RE(in, j + q * m), *si = IM(in, j + q * m);
+      double wre = 1.0, wim = 0.0;
       if (q) {
-        int idx = (q * k * tstep) % p->n;
-        cplx w = p->w[idx];
-        if (sign > 0) w.im = -w.im;
-        v = cmul(v, w);
+        wre = tw[k * (r - 1) + q - 1].re;
+        wim = -sign * tw[k * (r - 1) + q - 1].im; /* conj for the inverse */
       }
-      t[q] = v;
+      for (int b = 0; b < OR_VB; ++b) {
+        xr[q * OR_VB + b] = sr[b] * wre - si[b] * wim;
+        xi[q * OR_VB + b] = sr[b] * wim + si[b] * wre;
+      }
     }
-    for (int s = 0; s < r; ++s) {
-      cplx acc = {0.0, 0.0};
-      for (int q = 0; q < r; ++q) {
-        int idx = ((q * s) % r) * (p->n / r);
-        cplx w = p->w[idx];
-        if (sign > 0) w.im = -w.im;
-        cplx v = cmul(t[q], w);
-        acc.re += v.re;
-        acc.im += v.im;
+    if (r == 2) {
+      double *a0r = RE(out, o0), *a0i = IM(out, o0), *a1r = RE(out, o0 + ns), *a1i = IM(out, o0 + ns);
+      for (int b = 0; b < OR_VB; ++b) {
+        a0r[b] = xr[b] + xr[OR_VB + b];
+        a0i[b] = xi[b] + xi[OR_VB + b];
+        a1r[b] = xr[b] - xr[OR_VB + b];
+        a1i[b] = xi[b] - xi[OR_VB + b];
       }
-      out[s * m + k] = acc;
+    } else if (r == 4) {
+      /* y0 = (x0 + x2) + (x1 + x3), y2 = (x0 + x2) - (x1 + x3),
+         y1 = (x0 - x2) + sign i (x1 - x3), y3 = (x0 - x2) - sign i (x1 - x3) */
+      double *y0r = RE(out, o0), *y0i = IM(out, o0);
+      double *y1r = RE(out, o0 + ns), *y1i = IM(out, o0 + ns);
+      double *y2r = RE(out, o0 + 2 * ns), *y2i = IM(out, o0 + 2 * ns);
+      double *y3r = RE(out, o0 + 3 * ns), *y3i = IM(out, o0 + 3 * ns);
+      for (int b = 0; b < OR_VB; ++b) {
+        const double ar = xr[b] + xr[2 * OR_VB + b], ai = xi[b] + xi[2 * OR_VB + b];
+        const double cr = xr[b] - xr[2 * OR_VB + b], ci = xi[b] - xi[2 * OR_VB + b];
+        const double br = xr[OR_VB + b] + xr[3 * OR_VB + b], bi = xi[OR_VB + b] + xi[3 * OR_VB + b];
+        const double dr = xr[OR_VB + b] - xr[3 * OR_VB + b], di = xi[OR_VB + b] - xi[3 * OR_VB + b];
+        /* sign i (dr + i di) = (-sign di) + i (sign dr) */
+        const double er = -sign * di, ei = sign * dr;
+        y0r[b] = ar + br;
+        y0i[b] = ai + bi;
+        y2r[b] = ar - br;
+        y2i[b] = ai - bi;
+        y1r[b] = cr + er;
+        y1i[b] = ci + ei;
+        y3r[b] = cr - er;
+        y3i[b] = ci - ei;
+      }
+    } else {
+      /* the definition: y_t = sum_q x_q e^{sign 2 pi i q t / r} */
+      for (int t = 0; t < r; ++t) {
+        double *yr = RE(out, o0 + t * ns), *yi = IM(out, o0 + t * ns);
+        for (int b = 0; b < OR_VB; ++b) {
+          yr[b] = 0.0;
+          yi[b] = 0.0;
+        }
+        for (int q = 0; q < r; ++q) {
+          const cplx w = wr[(q * t) % r];
+          const double wre = w.re, wim = -sign * w.im;
+          for (int b = 0; b < OR_VB; ++b) {
+            yr[b] += xr[q * OR_VB + b] * wre - xi[q * OR_VB + b] * wim;
+            yi[b] += xr[q * OR_VB + b] * wim + xi[q * OR_VB + b] * wre;
+          }
+        }
+      }
     }
   }
+  free(gr);
+  free(gi);
 }
 
-/* unnormalised DFT of one contiguous line (in place) */
-static void fft_line(const fftplan *p, cplx *line, int sign, cplx *tmp, cplx *scratch) {
-  fft_rec(p, 0, p->n, line, 1, tmp, sign, scratch);
-  memcpy(line, tmp, sizeof(cplx) * (size_t)p->n);
+/* unnormalised DFT of the OR_VB lines of a block; returns the buffer that
+ * holds the result (blk or tmp) */
+static double *fft_block(const fftplan *p, double *blk, double *tmp, int sign) {
+  double *a = blk, *b = tmp;
+  int ns = 1;
+  for (int st = 0; st < p->nf; ++st) {
+    stage(p, st, ns, a, b, sign);
+    ns *= p->fac[st];
+    double *t = a;
+    a = b;
+    b = t;
+  }
+  return a;
 }
 
 /*
@@ -165,37 +244,56 @@ void or_fft3d(double *data, int nx, int ny, int nz, double bx, double by, double
       }
     }
     size_t nlines = ntot / (size_t)n;
+    size_t nblocks = (nlines + OR_VB - 1) / OR_VB;
 #pragma omp parallel
     {
-      cplx *line = (cplx *)malloc(sizeof(cplx) * n);
-      cplx *tmp = (cplx *)malloc(sizeof(cplx) * n);
-      cplx *scr = (cplx *)malloc(sizeof(cplx) * (size_t)n * 4 + 64);
+      double *blk = (double *)malloc(sizeof(double) * (size_t)n * 2 * OR_VB);
+      double *tmp = (double *)malloc(sizeof(double) * (size_t)n * 2 * OR_VB);
+      size_t base[OR_VB];
 #pragma omp for schedule(static)
-      for (size_t l = 0; l < nlines; ++l) {
-        size_t base;
-        if (ax == 0) {
-          base = l * (size_t)n;
-        } else if (ax == 1) {
-          size_t x = l % nx, z = l / nx;
-          base = x + z * (size_t)nx * ny;
-        } else {
-          base = l; /* l = x + y*nx */
+      for (size_t bl = 0; bl < nblocks; ++bl) {
+        /* lines bl*OR_VB .. +OR_VB-1; for y and z these are neighbouring x */
+        int nb = 0;
+        for (int b = 0; b < OR_VB; ++b) {
+          size_t l = bl * OR_VB + b;
+          if (l >= nlines) break;
+          if (ax == 0) {
+            base[b] = l * (size_t)n;
+          } else if (ax == 1) {
+            size_t x = l % nx, z = l / nx;
+            base[b] = x + z * (size_t)nx * ny;
+          } else {
+            base[b] = l; /* l = x + y*nx */
+          }
+          ++nb;
         }
-        for (int i = 0; i < n; ++i) line[i] = a[base + (size_t)i * st];
-        if (sign < 0) {
-          for (int i = 0; i < n; ++i) line[i] = cmul(line[i], pre[i]);
-          fft_line(&plan, line, -1, tmp, scr);
-          for (int i = 0; i < n; ++i) line[i] = cmul(line[i], post[i]);
-        } else {
-          for (int i = 0; i < n; ++i) line[i] = cmul(line[i], post[i]);
-          fft_line(&plan, line, +1, tmp, scr);
-          for (int i = 0; i < n; ++i) line[i] = cmul(line[i], pre[i]);
+        /* gather with the pre-twiddle (forward) or post-twiddle (inverse) */
+        const cplx *tin = (sign < 0) ? pre : post;
+        const cplx *tout = (sign < 0) ? post : pre;
+        for (int i = 0; i < n; ++i) {
+          double *br = RE(blk, i), *bi = IM(blk, i);
+          for (int b = 0; b < OR_VB; ++b) {
+            if (b < nb) {
+              cplx v = cmul(a[base[b] + (size_t)i * st], tin[i]);
+              br[b] = v.re;
+              bi[b] = v.im;
+            } else {
+              br[b] = 0.0;
+              bi[b] = 0.0;
+            }
+          }
         }
-        for (int i = 0; i < n; ++i) a[base + (size_t)i * st] = line[i];
+        double *res = fft_block(&plan, blk, tmp, sign);
+        for (int i = 0; i < n; ++i) {
+          const double *rr = RE(res, i), *ri = IM(res, i);
+          for (int b = 0; b < nb; ++b) {
+            cplx v = {rr[b], ri[b]};
+            a[base[b] + (size_t)i * st] = cmul(v, tout[i]);
+          }
+        }
       }
-      free(line);
+      free(blk);
       free(tmp);
-      free(scr);
     }
     free(pre);
     free(post);
@@ -358,13 +456,17 @@ static inline double rcomp(int j, int n, double d) { return (double)(j - n / 2) 
  * s^2 = max(|r - r_u|^2 / sig_uv^2, 4e-6); sig_uv = lam1 (sig_u + sig_v)/2,
  * eps_uv = sqrt(eps_u eps_v), beta = 1/(N_A k_B T).  cal_LJ.cu:26-80.
  */
-void or_lj(int nx, int ny, int nz, double bx, double by, double bz, int natu,
-           const double *xyz, const double *sigu, const double *epsu, int natv,
-           const double *sigv, const double *epsv, double temper, double lam1, double *u) {
+/* or_lj_rows: the same on the sub-block of rows z in [z0, z0 + nzs),
+ * y in [y0, y0 + nys) of the full grid (every x), u[v][(z - z0)][(y - y0)][x];
+ * or_lj is the whole grid. */
+void or_lj_rows(int nx, int ny, int nz, double bx, double by, double bz, int z0, int nzs,
+                int y0, int nys, int natu, const double *xyz, const double *sigu,
+                const double *epsu, int natv, const double *sigv, const double *epsv,
+                double temper, double lam1, double *u) {
   const double cut2 = 2.0e-3 * 2.0e-3;
   double dx = bx / nx, dy = by / ny, dz = bz / nz;
   double ikbt = 1.0 / (OR_AVOGADRO * OR_BOLTZMANN * temper);
-  size_t ntot = (size_t)nx * ny * nz;
+  size_t ntot = (size_t)nx * nys * nzs;
   double *su = (double *)malloc(sizeof(double) * natu * natv);
   double *eu = (double *)malloc(sizeof(double) * natu * natv);
   for (int v = 0; v < natv; ++v)
@@ -375,7 +477,8 @@ void or_lj(int nx, int ny, int nz, double bx, double by, double bz, int natu,
   for (int v = 0; v < natv; ++v) {
 #pragma omp parallel for schedule(static)
     for (size_t ip = 0; ip < ntot; ++ip) {
-      int x = (int)(ip % nx), y = (int)((ip / nx) % ny), z = (int)(ip / ((size_t)nx * ny));
+      int x = (int)(ip % nx), y = y0 + (int)((ip / nx) % nys),
+          z = z0 + (int)(ip / ((size_t)nx * nys));
       double rx = rcomp(x, nx, dx), ry = rcomp(y, ny, dy), rz = rcomp(z, nz, dz);
       double acc = 0.0;
       for (int a = 0; a < natu; ++a) {
@@ -394,6 +497,13 @@ void or_lj(int nx, int ny, int nz, double bx, double by, double bz, int natu,
   free(eu);
 }
 
+void or_lj(int nx, int ny, int nz, double bx, double by, double bz, int natu,
+           const double *xyz, const double *sigu, const double *epsu, int natv,
+           const double *sigv, const double *epsv, double temper, double lam1, double *u) {
+  or_lj_rows(nx, ny, nz, bx, by, bz, 0, nz, 0, ny, natu, xyz, sigu, epsu, natv, sigv, epsv,
+             temper, lam1, u);
+}
+
 /* ubeta = E_h a0 / (k_B T) * lam2, in Angstrom (cal_Coulomb.cu:31-33) */
 double or_ubeta(double temper, double lam2) {
   return OR_HARTREE * OR_BOHR / (OR_BOLTZMANN * temper) * lam2;
@@ -403,13 +513,15 @@ double or_ubeta(double temper, double lam2) {
  * or_coulomb: de = ubeta sum_{r>=1e-5} q/r;  fr = ubeta sum [r>=1e-5 ?
  * q (1 - e^{-r})/r : q].  cal_Coulomb.cu:40-62, 82-87.
  */
-void or_coulomb(int nx, int ny, int nz, double bx, double by, double bz, int natu,
-                const double *xyz, const double *q, double ubeta, double *de, double *fr) {
+void or_coulomb_rows(int nx, int ny, int nz, double bx, double by, double bz, int z0, int nzs,
+                     int y0, int nys, int natu, const double *xyz, const double *q,
+                     double ubeta, double *de, double *fr) {
   double dx = bx / nx, dy = by / ny, dz = bz / nz;
-  size_t ntot = (size_t)nx * ny * nz;
+  size_t ntot = (size_t)nx * nys * nzs;
 #pragma omp parallel for schedule(static)
   for (size_t ip = 0; ip < ntot; ++ip) {
-    int x = (int)(ip % nx), y = (int)((ip / nx) % ny), z = (int)(ip / ((size_t)nx * ny));
+    int x = (int)(ip % nx), y = y0 + (int)((ip / nx) % nys),
+        z = z0 + (int)(ip / ((size_t)nx * nys));
     double rx = rcomp(x, nx, dx), ry = rcomp(y, ny, dy), rz = rcomp(z, nz, dz);
     double e = 0.0, f = 0.0;
     for (int a = 0; a < natu; ++a) {
@@ -428,16 +540,23 @@ void or_coulomb(int nx, int ny, int nz, double bx, double by, double bz, int nat
   }
 }
 
+void or_coulomb(int nx, int ny, int nz, double bx, double by, double bz, int natu,
+                const double *xyz, const double *q, double ubeta, double *de, double *fr) {
+  or_coulomb_rows(nx, ny, nz, bx, by, bz, 0, nz, 0, ny, natu, xyz, q, ubeta, de, fr);
+}
+
 /*
  * or_fk: fk(k) = ubeta sum_u 4 pi q_u e^{-i k.r_u} / (k^2 (k^2 + 1)) on the
  * full k-grid (complex, interleaved).  cal_Coulomb.cu:65-79, 90-95.
  */
-void or_fk(int nx, int ny, int nz, double bx, double by, double bz, int natu,
-           const double *xyz, const double *q, double ubeta, double *fk) {
-  size_t ntot = (size_t)nx * ny * nz;
+void or_fk_rows(int nx, int ny, int nz, double bx, double by, double bz, int z0, int nzs,
+                int y0, int nys, int natu, const double *xyz, const double *q, double ubeta,
+                double *fk) {
+  size_t ntot = (size_t)nx * nys * nzs;
 #pragma omp parallel for schedule(static)
   for (size_t ip = 0; ip < ntot; ++ip) {
-    int x = (int)(ip % nx), y = (int)((ip / nx) % ny), z = (int)(ip / ((size_t)nx * ny));
+    int x = (int)(ip % nx), y = y0 + (int)((ip / nx) % nys),
+        z = z0 + (int)(ip / ((size_t)nx * nys));
     double kx = kcomp(x, nx, bx), ky = kcomp(y, ny, by), kz = kcomp(z, nz, bz);
     double rk2 = kx * kx + ky * ky + kz * kz;
     double rk4i = 1.0 / (rk2 * (rk2 + 1.0));
@@ -451,6 +570,11 @@ void or_fk(int nx, int ny, int nz, double bx, double by, double bz, int natu,
     fk[2 * ip] = re * ubeta;
     fk[2 * ip + 1] = im * ubeta;
   }
+}
+
+void or_fk(int nx, int ny, int nz, double bx, double by, double bz, int natu,
+           const double *xyz, const double *q, double ubeta, double *fk) {
+  or_fk_rows(nx, ny, nz, bx, by, bz, 0, nz, 0, ny, natu, xyz, q, ubeta, fk);
 }
 
 /* ------------------------------------------------------------------------- */

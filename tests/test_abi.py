@@ -150,3 +150,19 @@ def test_cli_old_version(golden, tmp_path):
     (tmp_path / "old.inp").write_text(" m 0 20200101\n KH\n")
     p = subprocess.run([exe, "old.inp"], cwd=tmp_path, capture_output=True, text=True)
     assert p.returncode == 1 and "old version" in p.stdout
+
+
+def test_fft_lengths_outside_the_built_set_are_refused(rism):
+    """Every even size up to 1024 that is not an instantiated FFT length
+    (test_gpu_fft_lengths.LENGTHS, each checked against the oracle on the
+    GPU) is refused with RISM_ERR_ARG before any device call."""
+    from test_gpu_fft_lengths import LENGTHS
+    L = rism.lib()
+    box = np.array([4.0, 4.0, 8.0])
+    for n in range(8, 1025, 2):
+        if n in LENGTHS:
+            continue
+        buf = np.zeros(n * 8 * 16)
+        rc = L.rism_k_fft_r2c(n, 8, 16, rism._ptr(box), rism._ptr(buf),
+                              rism._ptr(np.zeros(n * 8 * 8 * 2)))
+        assert rc == rism.RISM_ERR_ARG, n

@@ -121,3 +121,42 @@ def test_single_atom_closed_form_and_guards(oracle, golden):
         sr6 = (su / r) ** 6
         want = beta * 4 * eu * (sr6 * sr6 - sr6) + s.q[v] * ub * q / r
         assert u[v * N + i1] == pytest.approx(want, rel=1e-13)
+
+
+def test_rows_equal_full_grid(oracle, golden):
+    """potentials_rows (the oracle on a block of rows, for the full-size GPU
+    checks) returns exactly the matching block of potentials() on the whole
+    grid: the same loop over the same points."""
+    s = _solvent(oracle, golden)
+    grid, box = (12, 10, 16), (6.0, 5.0, 8.0)
+    rng = np.random.default_rng(3)
+    sol = oracle.Solute(rng.uniform(2.5, 3.5, 5), rng.uniform(60, 800, 5),
+                        rng.normal(0, 0.4, 5), rng.uniform(-2, 2, (5, 3)))
+    u, fr, fk = oracle.potentials(grid, box, s, sol)
+    nx, ny, nz = grid
+    z0, nzs, y0, nys = 5, 3, 2, 4
+    ub, frb, fkb = oracle.potentials_rows(grid, box, s, sol, z0, nzs, y0, nys)
+    u4 = u.reshape(s.natv, nz, ny, nx)
+    assert np.array_equal(ub, u4[:, z0:z0 + nzs, y0:y0 + nys])
+    assert np.array_equal(frb, fr.reshape(nz, ny, nx)[z0:z0 + nzs, y0:y0 + nys])
+    fkc = fk.view(np.complex128).reshape(nz, ny, nx)
+    assert np.array_equal(fkb, fkc[z0:z0 + nzs, y0:y0 + nys])
+
+
+def test_cubic_shells_equal_sorted_shells(oracle):
+    """cubic_shells (closed form, for 512^3 where sorting is slow) gives every
+    point the same |k| as kshells (the sort of initialize_g.cu) to rounding."""
+    for n, L in ((16, 8.0), (24, 11.0)):
+        ga, ind = oracle.kshells((n, n, n), (L, L, L))
+        ga2, ind2 = oracle.cubic_shells((n, n, n), (L, L, L))
+        assert np.abs(ga[ind] - ga2[ind2]).max() <= 1e-15 * ga.max()
+
+
+def test_full_spectrum_symmetry(oracle):
+    """full_spectrum rebuilds the whole transform of a real field from its
+    k_z > 0 half (F(-k) = conj F(k), k_{N-1-j} = -k_j)."""
+    rng = np.random.default_rng(4)
+    grid, box = (8, 12, 10), (4.0, 6.0, 5.0)
+    f = rng.standard_normal((10, 12, 8))
+    F = oracle.fft3d(f, box, -1)
+    assert np.abs(oracle.full_spectrum(F[5:]) - F).max() <= 1e-14 * np.abs(F).max()
