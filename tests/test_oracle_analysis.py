@@ -2,8 +2,9 @@
 cal_ad2), pinned by what the mathematics fixes rather than by retyping the
 integrands: grad is the derivative of the g-weighted interaction energy with
 respect to the atom's position (central differences), ad1 is the derivative
-of the LJ part with respect to the sigma scaling lambda, and ad2 is the
-Coulomb part of euv."""
+of the LJ part with respect to the sigma scaling lambda, ad2 is the
+Coulomb part of euv, and euv itself reduces to one pair energy (closed
+form, CODATA constants) when g is non-zero at a single grid point."""
 import math
 
 import numpy as np
@@ -95,3 +96,42 @@ def test_cutoff_excludes_the_atom_site(oracle):
     sol0 = oracle.Solute(sol.sig, sol.eps, sol.q, np.array([[0.0, 0.0, 0.0], [1.0, 0.5, 0.5]]))
     for kind in (0, 1, 2):
         assert np.all(np.isfinite(oracle.atom_sums(kind, grid, box, solv, sol0, g)))
+
+
+def test_euv_single_point_closed_form(oracle):
+    """g = 1 at two grid points and 0 elsewhere, one point inside sigma_uv / 2
+    of the atom (left out) and one outside: euv reduces to dv rho_v E_uv(r),
+    the pair energy of one solvent site at distance r, E_uv = 4 eps_uv
+    [(sigma_uv / r)^12 - (sigma_uv / r)^6] + q_u q_v e^2 / (4 pi eps0 r) per
+    mole, with the Lorentz-Berthelot pair parameters and the Coulomb constant
+    from scipy's CODATA values (the oracle's Hartree x Bohr x N_A, from the
+    reference's physical.h, agrees with them to 2e-7).  A wrong volume
+    element, density weight, unit or mixing rule changes the result by far
+    more than the tolerance."""
+    from scipy import constants as C
+
+    ke = C.e ** 2 / (4.0 * math.pi * C.epsilon_0) * C.N_A * 1e10  # J A / mol
+    solv, sol, _, _, _ = _setup(oracle)
+    sol1 = oracle.Solute(sol.sig[:1], sol.eps[:1], sol.q[:1], np.array([[0.1, -0.2, 0.05]]))
+    grid, box = (16, 16, 16), (8.0, 8.0, 8.0)
+    nx, ny, nz = grid
+    dv = 0.125
+    N = nx * ny * nz
+    p_in, p_out = (8, 8, 8), (12, 8, 5)  # (x, y, z) grid indices
+    g = np.zeros((solv.natv, N))
+    for x, y, z in (p_in, p_out):
+        g[:, (z * ny + y) * nx + x] = 1.0
+    got = oracle.atom_sums(1, grid, box, solv, sol1, g)[0]
+    r = np.linalg.norm(np.array([(p_out[0] - 8) * 0.5, (p_out[1] - 8) * 0.5,
+                                 (p_out[2] - 8) * 0.5]) - sol1.xyz[0])
+    r_in = np.linalg.norm(sol1.xyz[0])
+    exp_site = []
+    for v in range(solv.natv):
+        sg = 0.5 * (sol1.sig[0] + solv.sig[v])
+        assert r_in < 0.5 * sg <= r
+        eps = math.sqrt(sol1.eps[0] * solv.eps[v])
+        e_lj = 4.0 * eps * ((sg / r) ** 12 - (sg / r) ** 6)
+        e_c = ke * sol1.q[0] * solv.q[v] / r
+        exp_site.append(dv * solv.rho[v] * (e_lj + e_c))
+    assert got[1:] == pytest.approx(exp_site, rel=1e-6)
+    assert got[0] == pytest.approx(sum(exp_site), rel=1e-6)
