@@ -1,5 +1,5 @@
 // rism_common.h -- device helpers and compile-time tables shared by the
-// kernel translation units (rism_kzpass.hip, rism_kypass.hip,
+// kernel translation units (rism_kzpass64.hip, rism_kzpass32.hip, rism_kypass.hip,
 // rism_kxpass.hip, rism_kmisc.hip).
 #pragma once
 #include <hip/hip_runtime.h>

@@ -15,6 +15,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace rism {
 
 // Complex value of scalar type Re (double: the FP64 path; float: the FP32
@@ -210,6 +212,20 @@ struct FftShape {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
 
+// Staged global access of the first and last stages.  A plain Load is a
+// callable load(i) -> complex and a plain Store a callable store(i, v).  A
+// staged Load (with a member type `staged`) splits each element into
+// fetch(i) -> raw_t, its global loads, and make(raw, i) -> complex, the
+// arithmetic on them; a staged Store splits each output into prep(i) ->
+// aux_t, the global loads it needs, and put(i, v, aux).  The stage issues
+// the fetches (preps) of `group` elements of a butterfly before their makes
+// (puts), so those loads are in flight together instead of one dependent
+// load-compute chain per element; `group` bounds the registers they hold.
+template <class T, class = void>
+struct is_staged : std::false_type {};
+template <class T>
+struct is_staged<T, std::void_t<typename T::staged>> : std::true_type {};
+
 // One Stockham stage.  lds = this column's LDS region; tw = exp(-2 pi i q / L).
 template <int L, int R, int NS, int S, int T, bool FIRST, bool LAST, class Re, class Load,
           class Store>
@@ -219,17 +235,40 @@ __device__ __forceinline__ void stockham_stage(cx<Re> *lds, int tid,
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
   cx<Re> v[PER][R];
+  if constexpr (FIRST && is_staged<Load>::value) {
+    // Load::group elements are fetched together (bounded by the registers
+    // their raw values take)
+    constexpr int G = (int)Load::group;
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+      const int j = tid + b * T;
+      if ((NB % T == 0) || j < NB) {
+#pragma unroll
+        for (int r0 = 0; r0 < R; r0 += G) {
+          typename Load::raw_t raw[G];
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            if (r0 + g < R) raw[g] = load.fetch(j + (r0 + g) * NB);
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            if (r0 + g < R) v[b][r0 + g] = load.make(raw[g], j + (r0 + g) * NB);
+        }
+      }
+    }
+  }
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
     const int j = tid + b * T;
     if ((NB % T == 0) || j < NB) {
+      if constexpr (!(FIRST && is_staged<Load>::value)) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int idx = j + r * NB;
-        if constexpr (FIRST)
-          v[b][r] = load(idx);
-        else
-          v[b][r] = lds[lpad(idx)];
+        for (int r = 0; r < R; ++r) {
+          const int idx = j + r * NB;
+          if constexpr (FIRST)
+            v[b][r] = load(idx);
+          else
+            v[b][r] = lds[lpad(idx)];
+        }
       }
       if constexpr (NS > 1) {
         const int k = j % NS;
@@ -249,13 +288,27 @@ __device__ __forceinline__ void stockham_stage(cx<Re> *lds, int tid,
     if ((NB % T == 0) || j < NB) {
       const int k = j % NS;
       const int idxD = (j / NS) * NS * R + k;
+      if constexpr (LAST && is_staged<Store>::value) {
+        constexpr int G = (int)Store::group;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int idx = idxD + r * NS;
-        if constexpr (LAST)
-          store(idx, v[b][r]);
-        else
-          lds[lpad(idx)] = v[b][r];
+        for (int r0 = 0; r0 < R; r0 += G) {
+          typename Store::aux_t aux[G];
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            if (r0 + g < R) aux[g] = store.prep(idxD + (r0 + g) * NS);
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            if (r0 + g < R) store.put(idxD + (r0 + g) * NS, v[b][r0 + g], aux[g]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int idx = idxD + r * NS;
+          if constexpr (LAST)
+            store(idx, v[b][r]);
+          else
+            lds[lpad(idx)] = v[b][r];
+        }
       }
     }
   }

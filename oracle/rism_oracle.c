@@ -39,6 +39,50 @@
 #define OR_BOLTZMANN 1.380649e-23
 #define OR_HARTREE 4.359744722207185e-18
 
+/*
+ * Compensated (Neumaier) accumulation for the long sums over the grid (rms,
+ * Gram entries, thermodynamic integrals).  The reference reduces these with
+ * thrust::reduce, a tree whose rounding error does not grow with the length;
+ * a plain running sum over 10^8 terms per thread drifts by ~1e-11 relative,
+ * which is what the check of a 512^3 step would then see.  Each thread keeps
+ * OR_NSUM compensated sums over its static chunk; the per-thread results are
+ * added in thread order, so a run is reproducible for a fixed thread count.
+ */
+#define OR_NSUM 5
+#define OR_MAXTHR 1024
+typedef struct { double s[OR_NSUM], c[OR_NSUM]; } or_ksum;
+static inline void ks_add(or_ksum *k, int j, double x) {
+  const double t = k->s[j] + x;
+  if (fabs(k->s[j]) >= fabs(x)) k->c[j] += (k->s[j] - t) + x;
+  else k->c[j] += (x - t) + k->s[j];
+  k->s[j] = t;
+}
+static inline int or_thread_id(void) {
+#ifdef _OPENMP
+  return omp_get_thread_num();
+#else
+  return 0;
+#endif
+}
+static inline int or_thread_count(void) {
+#ifdef _OPENMP
+  int n = omp_get_max_threads();
+  return n < OR_MAXTHR ? n : OR_MAXTHR;
+#else
+  return 1;
+#endif
+}
+/* sums the per-thread partials part[0..nt) of component j, in order */
+static double ks_total(const or_ksum *part, int nt, int j) {
+  or_ksum k;
+  memset(&k, 0, sizeof(k));
+  for (int i = 0; i < nt; ++i) {
+    ks_add(&k, j, part[i].s[j]);
+    ks_add(&k, j, part[i].c[j]);
+  }
+  return k.s[j] + k.c[j];
+}
+
 typedef struct { double re, im; } cplx;
 
 static inline cplx cmul(cplx a, cplx b) {
@@ -655,14 +699,22 @@ double or_calculate(int nx, int ny, int nz, double bx, double by, double bz, int
     or_fft3d(hv, nx, ny, nz, bx, by, bz, +1);
     const double *gv = g + (size_t)v * N;
     double *Rv = R + (size_t)v * N, *hr = h + (size_t)v * N;
-    double s = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : s)
-    for (size_t i = 0; i < N; ++i) { /* tr */
-      hr[i] = hv[2 * i];
-      Rv[i] = hv[2 * i] + 1.0 - gv[i];
-      s += Rv[i] * Rv[i];
+    or_ksum part[OR_MAXTHR];
+    const int nt = or_thread_count();
+    memset(part, 0, sizeof(or_ksum) * nt);
+#pragma omp parallel num_threads(nt)
+    {
+      or_ksum k;
+      memset(&k, 0, sizeof(k));
+#pragma omp for schedule(static)
+      for (size_t i = 0; i < N; ++i) { /* tr */
+        hr[i] = hv[2 * i];
+        Rv[i] = hv[2 * i] + 1.0 - gv[i];
+        ks_add(&k, 0, Rv[i] * Rv[i]);
+      }
+      part[or_thread_id()] = k;
     }
-    ss += s;
+    ss += ks_total(part, nt, 0);
   }
   return sqrt(ss / ((double)N * natv));
 }
@@ -739,18 +791,29 @@ void or_an2_step(or_an2 *A, double *t, const double *R) {
     double c[2] = {0, 0}, a[4] = {0, 0, 0, 0};
     for (int v = 0; v < A->natv; ++v) {
       size_t off = (size_t)v * A->N;
-      double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-#pragma omp parallel for schedule(static) reduction(+ : s0, s1, s2, s3, s4)
-      for (size_t i = 0; i < A->N; ++i) {
-        double r = R[off + i];
-        double t1 = r - A->rp[off + i];
-        double t2 = r - A->rp[nn + off + i];
-        s0 += r * t1;
-        s1 += r * t2;
-        s2 += t1 * t1;
-        s3 += t1 * t2;
-        s4 += t2 * t2;
+      or_ksum part[OR_MAXTHR];
+      const int nt = or_thread_count();
+      memset(part, 0, sizeof(or_ksum) * nt);
+#pragma omp parallel num_threads(nt)
+      {
+        or_ksum k;
+        memset(&k, 0, sizeof(k));
+#pragma omp for schedule(static)
+        for (size_t i = 0; i < A->N; ++i) {
+          double r = R[off + i];
+          double t1 = r - A->rp[off + i];
+          double t2 = r - A->rp[nn + off + i];
+          ks_add(&k, 0, r * t1);
+          ks_add(&k, 1, r * t2);
+          ks_add(&k, 2, t1 * t1);
+          ks_add(&k, 3, t1 * t2);
+          ks_add(&k, 4, t2 * t2);
+        }
+        part[or_thread_id()] = k;
       }
+      const double s0 = ks_total(part, nt, 0), s1 = ks_total(part, nt, 1),
+                   s2 = ks_total(part, nt, 2), s3 = ks_total(part, nt, 3),
+                   s4 = ks_total(part, nt, 4);
       c[0] += s0 * A->irho[v];
       c[1] += s1 * A->irho[v];
       a[0] += s2 * A->irho[v];
@@ -805,18 +868,28 @@ void or_thermo(int natv, size_t N, double dv, int clos, const double *rhov,
   double cu = 0.0;
   for (int v = 0; v < natv; ++v) {
     const double *hv = h + (size_t)v * N, *tv = t + (size_t)v * N;
-    double sc = 0, sh = 0, gf = 0, pm = 0;
-#pragma omp parallel for schedule(static) reduction(+ : sc, sh, gf, pm)
-    for (size_t i = 0; i < N; ++i) {
-      double hh = hv[i], tt = tv[i];
-      double ehnc = hh * 0.5 * tt - (hh - tt);
-      double egf = -(hh * 0.5 + 1.0) * (hh - tt);
-      double ekh = (hh > 0.0) ? egf : ehnc;
-      sc += (clos == 0) ? ekh : ehnc;
-      sh += ehnc;
-      gf += egf;
-      pm += hh - tt;
+    or_ksum part[OR_MAXTHR];
+    const int nt = or_thread_count();
+    memset(part, 0, sizeof(or_ksum) * nt);
+#pragma omp parallel num_threads(nt)
+    {
+      or_ksum k;
+      memset(&k, 0, sizeof(k));
+#pragma omp for schedule(static)
+      for (size_t i = 0; i < N; ++i) {
+        double hh = hv[i], tt = tv[i];
+        double ehnc = hh * 0.5 * tt - (hh - tt);
+        double egf = -(hh * 0.5 + 1.0) * (hh - tt);
+        double ekh = (hh > 0.0) ? egf : ehnc;
+        ks_add(&k, 0, (clos == 0) ? ekh : ehnc);
+        ks_add(&k, 1, ehnc);
+        ks_add(&k, 2, egf);
+        ks_add(&k, 3, hh - tt);
+      }
+      part[or_thread_id()] = k;
     }
+    const double sc = ks_total(part, nt, 0), sh = ks_total(part, nt, 1),
+                 gf = ks_total(part, nt, 2), pm = ks_total(part, nt, 3);
     xmu[v] = sc * rhov[v] * dv;
     xmu[natv + v] = gf * rhov[v] * dv;
     xmu2[v] = (clos == 0) ? sh * rhov[v] * dv : 0.0;

@@ -41,7 +41,7 @@ def test_cli_output_files(rism, oracle, gpu, golden, tmp_path, ad, lam):
     os.symlink(os.path.join(golden, "h2o_vv.xvk"), tmp_path / "h2o_vv.xvk")
     sol = G.random_walk_solute(10, seed=77, total_charge=0.0)
     sol["xyz"] = sol["xyz"] + np.array([0.37, -0.21, 1.13])  # non-zero centring shift
-    grid, box = (40, 32, 36), (20.0, 16.0, 18.0)
+    grid, box = (40, 32, 48), (20.0, 16.0, 24.0)
     G.write_inp(tmp_path / "walk.inp", sol, "h2o_vv.xvk", grid, box, outlist="mgdab")
     cmd = [exe] + ({1: ["-l", str(lam)], 2: ["-e", str(lam)]}.get(ad, [])) + ["walk.inp"]
     p = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=300)
