@@ -176,15 +176,15 @@ struct rism_ctx {
   bool fresh = false;  // no rism_anderson since that rism_calculate
   // Anderson(m) state (opts.anderson_m > 0): residuals in Rb[m + 1], mixed
   // iterates P_p = t_p + beta R_p in Pb[m]; hist lists the (R, P) buffer
-  // pairs oldest first; D holds the weighted dots <R_a, R_b> by R buffer.
+  // pairs oldest first.  On the device: dD, the weighted dots <R_a, R_b> by
+  // R buffer ((m + 1)^2), and xcoef, the mixing coefficients of the next
+  // update (both written by the step's finish kernel).
   struct HEnt {
     int rb, pb;
   };
   std::vector<HEnt> hist;
   int cur_rb = 0, next_pb = 0;
-  std::vector<double> D;
-  std::vector<double> lastth;
-  std::vector<double> hsums;
+  double *dD = nullptr, *xcoef = nullptr;
   // device-side step state: scal = {rms, x0, x1, s1, s2} of the last step,
   // dflag = {done (1 converged, 2 diverged), step}, rms_hist[step]; hscal
   // and hflag are their pinned host mirrors
@@ -403,7 +403,6 @@ void reset_an2(rism_ctx *c) {
   c->hist.clear();
   c->cur_rb = 0;
   c->next_pb = 0;
-  c->lastth.clear();
   c->lastx[0] = c->lastx[1] = c->lasts[0] = c->lasts[1] = 0.0;
 }
 
@@ -454,71 +453,6 @@ struct KTime {
   }
 };
 
-// Coefficients of the Anderson(m) step from the weighted dots D: minimise
-// |R - sum_p th_p (R - R_p)| over the newest k history entries, i.e. solve
-// G th = b with G_pq = <R - R_p, R - R_q>, b_p = <R, R - R_p>, after
-// scaling G to unit diagonal, by Gaussian elimination with partial
-// pivoting.  When a pivot falls below 1e-12 (or a diagonal is not
-// positive) the oldest entry is left out and the solve repeated.
-// Returns th for all entries (zeros for the ones left out).
-std::vector<double> anderson_m_coeffs(const std::vector<double> &D, int m1, int cur,
-                                      const std::vector<int> &rb) {
-  const int nh = (int)rb.size();
-  std::vector<double> th(nh, 0.0);
-  const double rr = D[(size_t)cur * m1 + cur];
-  for (int first = 0; first < nh; ++first) {
-    const int k = nh - first;
-    std::vector<double> G((size_t)k * k), b(k), sc(k);
-    bool ok = true;
-    for (int p = 0; p < k && ok; ++p) {
-      const int bp = rb[first + p];
-      const double dp = D[(size_t)cur * m1 + bp];
-      b[p] = rr - dp;
-      for (int q = 0; q < k; ++q) {
-        const int bq = rb[first + q];
-        G[(size_t)p * k + q] = rr - dp - D[(size_t)cur * m1 + bq] + D[(size_t)bp * m1 + bq];
-      }
-      ok = G[(size_t)p * k + p] > 0.0;
-      if (ok) sc[p] = 1.0 / std::sqrt(G[(size_t)p * k + p]);
-    }
-    if (!ok) continue;
-    for (int p = 0; p < k; ++p) {
-      b[p] *= sc[p];
-      for (int q = 0; q < k; ++q) G[(size_t)p * k + q] *= sc[p] * sc[q];
-    }
-    std::vector<int> perm(k);
-    for (int p = 0; p < k; ++p) perm[p] = p;
-    for (int col = 0; col < k && ok; ++col) {
-      int piv = col;
-      for (int r = col + 1; r < k; ++r)
-        if (std::fabs(G[(size_t)r * k + col]) > std::fabs(G[(size_t)piv * k + col])) piv = r;
-      if (std::fabs(G[(size_t)piv * k + col]) < 1e-12) {
-        ok = false;
-        break;
-      }
-      if (piv != col) {
-        for (int q = 0; q < k; ++q) std::swap(G[(size_t)piv * k + q], G[(size_t)col * k + q]);
-        std::swap(b[piv], b[col]);
-      }
-      for (int r = col + 1; r < k; ++r) {
-        const double f = G[(size_t)r * k + col] / G[(size_t)col * k + col];
-        for (int q = col; q < k; ++q) G[(size_t)r * k + q] -= f * G[(size_t)col * k + q];
-        b[r] -= f * b[col];
-      }
-    }
-    if (!ok) continue;
-    std::vector<double> x(k);
-    for (int r = k - 1; r >= 0; --r) {
-      double a = b[r];
-      for (int q = r + 1; q < k; ++q) a -= G[(size_t)r * k + q] * x[q];
-      x[r] = a / G[(size_t)r * k + r];
-    }
-    for (int p = 0; p < k; ++p) th[first + p] = x[p] * sc[p];
-    return th;
-  }
-  return th;
-}
-
 // One Anderson(m) step: the first an2_count steps are the reference's
 // Picard steps t' = t + R (anderson.cu:54-60); after that
 //   t' = Q + sum_p th_p (P_p - Q),  Q = t + beta R,  beta = an2_m.
@@ -530,28 +464,25 @@ int anderson_m_step(rism_ctx *c) {
   const size_t nn = c->N * c->natv;
   const int m = c->opts.anderson_m, m1 = m + 1;
   Re *t = L.T[c->iT_cur], *tnew = L.T[1 - c->iT_cur];
+  (void)m1;
   Re *R = L.Rb[c->cur_rb];
   // slot for the new pair: the oldest when full, else a fresh one
   const bool full = (int)c->hist.size() == m;
   const rism_ctx::HEnt old = full ? c->hist.front() : rism_ctx::HEnt{-1, c->next_pb};
   Re *Pdst = L.Pb[old.pb];
+  const int *done = c->async ? c->dflag : nullptr;
   if (c->count > 0) {
     --c->count;
     KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_an2_picard(tnew, t, R, Pdst, c->opts.an2_m, nn, nullptr, c->st));
-    c->lastth.clear();
+    HIPCHECK(rism::run_an2_picard(tnew, t, R, Pdst, c->opts.an2_m, nn, done, c->st));
   } else {
-    std::vector<int> rb;
-    for (const auto &h : c->hist) rb.push_back(h.rb);
-    c->lastth = anderson_m_coeffs(c->D, m1, c->cur_rb, rb);
+    // coefficients from the step's finish kernel (run_andm_finish)
     rism::HistUpdateT<Re> H{};
     H.nh = (int)c->hist.size();
-    for (int p = 0; p < H.nh; ++p) {
-      H.p[p] = L.Pb[c->hist[p].pb];
-      H.x[p] = c->lastth[p];
-    }
+    for (int p = 0; p < H.nh; ++p) H.p[p] = L.Pb[c->hist[p].pb];
     KTime kt(c, RISM_K_AN2);
-    HIPCHECK(rism::run_andm_update(tnew, t, R, H, Pdst, c->opts.an2_m, nn, c->st));
+    HIPCHECK(rism::run_andm_update(tnew, t, R, H, Pdst, c->xcoef, c->opts.an2_m, nn, done,
+                                   c->st));
   }
   if (full) c->hist.erase(c->hist.begin());
   else ++c->next_pb;
@@ -649,11 +580,20 @@ int calc_step(rism_ctx *c, double *rms, bool lin = false, double tol = -1.0) {
   {
     KTime kt(c, RISM_K_REDUCE);
     HIPCHECK(rism::run_reduce_cols(c->partial, nb, W, c->natv, c->sums, P.done, c->st));
-    if (!andm)
+    if (andm) {
+      int rb[RISM_MAX_HIST];
+      const int nh = (int)c->hist.size();
+      for (int p = 0; p < nh; ++p) rb[p] = c->hist[p].rb;
+      HIPCHECK(rism::run_andm_finish(c->sums, c->rho_dev, c->natv, c->N, tol, W, c->dD,
+                                     c->opts.anderson_m + 1, c->cur_rb, rb, nh, c->count == 0,
+                                     c->xcoef, c->scal, c->async ? c->rms_hist : nullptr,
+                                     c->async ? c->dflag : nullptr, c->st));
+    } else {
       HIPCHECK(rism::run_step_finish(c->sums, c->rho_dev, c->natv, c->N, tol,
                                      c->count == 0, c->opts.an2_mp, c->binary, c->scal,
                                      c->async ? c->rms_hist : nullptr,
                                      c->async ? c->dflag : nullptr, c->st));
+    }
   }
   c->iT_last = c->iT_cur;
   L.Rlast = Rout;
@@ -662,34 +602,14 @@ int calc_step(rism_ctx *c, double *rms, bool lin = false, double tol = -1.0) {
   c->have_step = true;
   if (c->async) return RISM_OK;
   double ss = 0.0;
-  if (andm) {
-    HIPCHECK(hipMemcpyAsync(c->hsums.data(), c->sums, sizeof(double) * W * c->natv,
-                            hipMemcpyDeviceToHost, c->st));
-    HIPCHECK(hipStreamSynchronize(c->st));
-    for (int v = 0; v < c->natv; ++v) ss += c->hsums[W * v];
-    ss = std::sqrt(ss / ((double)c->N * c->natv));  // cal_rms.cu:24
-    // weighted dots of the new residual with itself and the history
-    const int m1 = c->opts.anderson_m + 1;
-    auto wdot = [&](int k) {
-      double a = 0.0;
-      for (int v = 0; v < c->natv; ++v) a += c->hsums[W * v + k] / c->rho[v];
-      return a;
-    };
-    c->D[(size_t)c->cur_rb * m1 + c->cur_rb] = wdot(0);
-    for (size_t p = 0; p < c->hist.size(); ++p) {
-      const int bb = c->hist[p].rb;
-      c->D[(size_t)c->cur_rb * m1 + bb] = c->D[(size_t)bb * m1 + c->cur_rb] = wdot(1 + (int)p);
-    }
-  } else {
-    HIPCHECK(hipMemcpyAsync(c->hscal, c->scal, sizeof(double) * 5, hipMemcpyDeviceToHost, c->st));
-    HIPCHECK(hipStreamSynchronize(c->st));
-    ss = c->hscal[0];
-    if (c->count == 0) {
-      c->lastx[0] = c->hscal[1];
-      c->lastx[1] = c->hscal[2];
-      c->lasts[0] = c->hscal[3];
-      c->lasts[1] = c->hscal[4];
-    }
+  HIPCHECK(hipMemcpyAsync(c->hscal, c->scal, sizeof(double) * 5, hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  ss = c->hscal[0];
+  if (!andm && c->count == 0) {
+    c->lastx[0] = c->hscal[1];
+    c->lastx[1] = c->hscal[2];
+    c->lasts[0] = c->hscal[3];
+    c->lasts[1] = c->hscal[4];
   }
   if (c->ktime) ktime_resolve(c);
   if (rms) *rms = ss;
@@ -891,7 +811,7 @@ int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, b
   return RISM_OK;
 }
 
-// ---- solves without a host round trip per step (FP64, AN2) -------------------
+// ---- solves without a host round trip per step (FP64, AN2 and Anderson(m)) ---
 // The host enqueues steps in chunks; each step's finish kernel writes
 // rms_hist[step] and raises dflag at convergence or divergence, after which
 // every enqueued kernel of the context returns at entry.  The host reads the
@@ -909,12 +829,14 @@ struct AsyncRun {
   double rms = 0.0;
   struct Log {  // the accelerator's indices when the step was enqueued
     int iT, iR, s0, s1, b, count;
+    int cur_rb, next_pb;                  // Anderson(m)
+    std::vector<rism_ctx::HEnt> hist;     // Anderson(m)
   };
   std::vector<Log> log;  // per step
 };
 
 bool async_capable(const rism_ctx *c) {
-  return c->opts.anderson_m == 0 && c->opts.precision == 0 && !c->opts.verbose;
+  return c->opts.precision == 0 && !c->opts.verbose;
 }
 
 int async_begin(AsyncRun &a, rism_ctx *c, int maxstep, double tol) {
@@ -922,7 +844,7 @@ int async_begin(AsyncRun &a, rism_ctx *c, int maxstep, double tol) {
   a.c = c;
   a.maxstep = maxstep;
   a.tol = tol;
-  a.log.assign(maxstep + 1, AsyncRun::Log{0, 0, 0, 0, 0, 0});
+  a.log.assign(maxstep + 1, AsyncRun::Log{0, 0, 0, 0, 0, 0, 0, 0, {}});
   if (c->rms_cap < (size_t)maxstep + 1) {
     size_t cap = std::max<size_t>(1024, (size_t)maxstep + 1);
     int rc = c->alloc(&c->rms_hist, cap);
@@ -990,17 +912,31 @@ int ensure_graph(rism_ctx *c, double tol) {
 }
 
 AsyncRun::Log index_log(const rism_ctx *c) {
-  return AsyncRun::Log{c->iT_cur, c->iR_cur, c->iR_slot[0], c->iR_slot[1], c->binary, c->count};
+  AsyncRun::Log L{c->iT_cur, c->iR_cur, c->iR_slot[0], c->iR_slot[1], c->binary, c->count,
+                  c->cur_rb, c->next_pb, {}};
+  if (c->opts.anderson_m > 0) L.hist = c->hist;
+  return L;
 }
 
 // enqueues up to `chunk` steps at level Re and the read-back of the flag:
-// Picard steps and remainders directly, whole periods of mixing steps as
-// graph launches
+// Picard steps and remainders directly, whole periods of AN2 mixing steps
+// as graph launches.  Anderson(m) steps are enqueued directly (their buffer
+// rotation repeats only every lcm(m + 1, m, 2) steps; the coefficients come
+// from the device-side finish kernel, so nothing is read back).
 template <class Re>
 int async_enqueue(AsyncRun &a, int chunk) {
   rism_ctx *c = a.c;
   int rc;
+  const bool andm = c->opts.anderson_m > 0;
   for (int k = 0; k < chunk && a.istep < a.maxstep;) {
+    if (andm) {
+      a.log[++a.istep] = index_log(c);
+      if ((rc = calc_step<Re>(c, nullptr, false, a.tol))) return rc;
+      c->fresh = false;
+      if ((rc = anderson_m_step<Re>(c))) return rc;
+      ++k;
+      continue;
+    }
     if (c->count == 0 && k + kGraphSteps <= chunk && a.istep + kGraphSteps <= a.maxstep) {
       if ((rc = ensure_graph<Re>(c, a.tol))) return rc;
       HIPCHECK(hipGraphLaunch(c->gexec, c->st));
@@ -1039,8 +975,9 @@ int async_check(AsyncRun &a) {
   a.rms = c->hscal[0];
   c->async = false;
   const AsyncRun::Log &L = a.log[k];
+  const bool andm = c->opts.anderson_m > 0;
   c->iT_last = L.iT;
-  lv<Re>(c).Rlast = lv<Re>(c).R[L.iR];
+  lv<Re>(c).Rlast = andm ? lv<Re>(c).Rb[L.cur_rb] : lv<Re>(c).R[L.iR];
   c->last_prec = c->prec;
   c->have_step = true;
   c->fresh = false;
@@ -1054,6 +991,11 @@ int async_check(AsyncRun &a) {
     c->iR_slot[1] = L.s1;
     c->binary = L.b;
     c->count = L.count;
+    if (andm) {
+      c->cur_rb = L.cur_rb;
+      c->next_pb = L.next_pb;
+      c->hist = L.hist;
+    }
   }
   return RISM_OK;
 }
@@ -1263,8 +1205,14 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
         return fail(rism::set_error(RISM_ERR_HIP, "chi table conversion failed"));
     }
   }
-  if (c->opts.anderson_m > 0)
-    c->D.assign((size_t)(c->opts.anderson_m + 1) * (c->opts.anderson_m + 1), 0.0);
+  if (c->opts.anderson_m > 0) {
+    const size_t m1 = (size_t)c->opts.anderson_m + 1;
+    if ((rc = c->alloc(&c->dD, m1 * m1))) return fail(rc);
+    if ((rc = c->alloc(&c->xcoef, RISM_MAX_HIST))) return fail(rc);
+    if (hipMemsetAsync(c->dD, 0, sizeof(double) * m1 * m1, c->st) != hipSuccess ||
+        hipMemsetAsync(c->xcoef, 0, sizeof(double) * RISM_MAX_HIST, c->st) != hipSuccess)
+      return fail(rism::set_error(RISM_ERR_HIP, "clear of the Anderson(m) state failed"));
+  }
   c->zinv_blocks = std::max(std::max(rism::zinv_num_blocks<double>(c->P, 0),
                                      rism::zinv_num_blocks<double>(c->P, 1)),
                             std::max(rism::zinv_num_blocks<float>(c->P, 0),
@@ -1274,7 +1222,6 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   if ((rc = c->alloc(&c->tpartial, (size_t)tb * 4 * c->natv))) return fail(rc);
   if ((rc = c->alloc(&c->sums, kSumWidth * RISM_MAX_SITES))) return fail(rc);
   if ((rc = c->alloc(&c->scratch, c->N))) return fail(rc);
-  c->hsums.assign(kSumWidth * c->natv, 0.0);
   if ((rc = c->alloc(&c->scal, 8))) return fail(rc);
   if ((rc = c->alloc(&c->dflag, 3))) return fail(rc);
   if ((rc = upload(c, &c->rho_dev, c->rho))) return fail(rc);

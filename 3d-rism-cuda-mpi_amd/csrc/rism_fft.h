@@ -20,9 +20,10 @@
 namespace rism {
 
 // Complex value of scalar type Re (double: the FP64 path; float: the FP32
-// bulk of the mixed-precision path).
+// bulk of the mixed-precision path), aligned to its size so that LDS and
+// global accesses are single 16-byte (8-byte) vector operations.
 template <class Re>
-struct cx {
+struct alignas(2 * sizeof(Re)) cx {
   Re x, y;
 };
 using cplx = cx<double>;

@@ -45,7 +45,6 @@ struct HistT {
 template <class Re>
 struct HistUpdateT {
   const Re *p[RISM_MAX_HIST];  // P_p = T_p + beta R_p, p < nh
-  double x[RISM_MAX_HIST];     // mixing coefficients
   int nh;
 };
 
@@ -90,9 +89,20 @@ template <class Re>
 hipError_t run_zinv_hist(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
                          const Re *r0, const HistT<Re> &H, double *partial, const cx<Re> *twM,
                          const cx<Re> *zpre, const cx<Re> *zpost, hipStream_t st);
+// xcoef: the nh mixing coefficients in device memory (run_andm_finish)
 template <class Re>
 hipError_t run_andm_update(Re *tnew, const Re *t, const Re *R, const HistUpdateT<Re> &H, Re *Pb,
-                           double beta, size_t n, hipStream_t st);
+                           const double *xcoef, double beta, size_t n, const int *done,
+                           hipStream_t st);
+// End of an Anderson(m) step on the device: rms, flags and rms_hist as
+// run_step_finish (done may be null); the dots of the residual in buffer
+// `cur` with itself and the history buffers rb[0..nh) (oldest first) into
+// D[m1][m1]; when mixing, the nh coefficients into xcoef.  sums: [site][width]
+// FP64, R.R then R.R_p.
+hipError_t run_andm_finish(const double *sums, const double *rho, int natv, size_t N, double tol,
+                           int width, double *D, int m1, int cur, const int *rb, int nh,
+                           int mixing, double *xcoef, double *scal, double *rms_hist, int *done,
+                           hipStream_t st);
 // nsets consecutive blocks of nrow x width partial sums -> nsets x width
 hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, double *out,
                            const int *done, hipStream_t st);

@@ -188,6 +188,120 @@ __global__ void k_step_finish(const double *__restrict__ sums, const double *__r
 }
 
 // ---------------------------------------------------------------------------
+// End of one Anderson(m) step on the device (no counterpart in the
+// reference): rms and the convergence / divergence flag as k_step_finish;
+// the weighted dots of the new residual R (buffer cur) with itself and the
+// nh history residuals (buffers rb[0..nh), oldest first) enter the dot matrix
+// D[m1][m1]; and, when the next update mixes, the coefficients th of
+//   min |R - sum_p th_p (R - R_p)|  over the newest k history entries:
+// G th = b with G_pq = <R - R_p, R - R_q>, b_p = <R, R - R_p>, G scaled to a
+// unit diagonal, Gaussian elimination with partial pivoting; when a pivot
+// falls below 1e-12 (or a diagonal is not positive) the oldest entry is left
+// out and the solve repeated.  th (zeros for entries left out) goes to
+// xcoef[0..nh).  One thread, FP64; W = 1 + RISM_MAX_HIST sums per site.
+// ---------------------------------------------------------------------------
+struct AndmHist {
+  int rb[RISM_MAX_HIST];
+  int nh;
+};
+
+__global__ void k_andm_finish(const double *__restrict__ sums, const double *__restrict__ rho,
+                              int natv, size_t N, double tol, int W, double *__restrict__ D,
+                              int m1, int cur, AndmHist H, int mixing,
+                              double *__restrict__ xcoef, double *__restrict__ scal,
+                              double *__restrict__ rms_hist, int *__restrict__ done) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (done && done[0]) return;
+  int step = 0;
+  if (done) {
+    step = done[2] + 1;
+    done[2] = step;
+  }
+  double ss = 0.0;
+  for (int v = 0; v < natv; ++v) ss += sums[W * v];
+  const double rms = sqrt(ss / ((double)N * natv));  // cal_rms.cu:24
+  scal[0] = rms;
+  if (rms_hist) rms_hist[step] = rms;
+  if (done) {
+    if (!isfinite(rms)) {
+      done[0] = 2;
+      done[1] = step;
+    } else if (rms <= tol) {
+      done[0] = 1;
+      done[1] = step;
+    }
+  }
+  // weighted dots <R, R> and <R, R_p>, weights 1 / rho_v (anderson.cu:37-39)
+  auto wdot = [&](int k) {
+    double a = 0.0;
+    for (int v = 0; v < natv; ++v) a += sums[W * v + k] / rho[v];
+    return a;
+  };
+  D[(size_t)cur * m1 + cur] = wdot(0);
+  for (int p = 0; p < H.nh; ++p) {
+    const int bb = H.rb[p];
+    D[(size_t)cur * m1 + bb] = D[(size_t)bb * m1 + cur] = wdot(1 + p);
+  }
+  if (!mixing) return;
+  const int nh = H.nh;
+  for (int p = 0; p < nh; ++p) xcoef[p] = 0.0;
+  const double rr = D[(size_t)cur * m1 + cur];
+  double G[RISM_MAX_HIST][RISM_MAX_HIST], b[RISM_MAX_HIST], sc[RISM_MAX_HIST], x[RISM_MAX_HIST];
+  for (int first = 0; first < nh; ++first) {
+    const int k = nh - first;
+    bool ok = true;
+    for (int p = 0; p < k && ok; ++p) {
+      const int bp = H.rb[first + p];
+      const double dp = D[(size_t)cur * m1 + bp];
+      b[p] = rr - dp;
+      for (int q = 0; q < k; ++q) {
+        const int bq = H.rb[first + q];
+        G[p][q] = rr - dp - D[(size_t)cur * m1 + bq] + D[(size_t)bp * m1 + bq];
+      }
+      ok = G[p][p] > 0.0;
+      if (ok) sc[p] = 1.0 / sqrt(G[p][p]);
+    }
+    if (!ok) continue;
+    for (int p = 0; p < k; ++p) {
+      b[p] *= sc[p];
+      for (int q = 0; q < k; ++q) G[p][q] *= sc[p] * sc[q];
+    }
+    for (int col = 0; col < k && ok; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < k; ++r)
+        if (fabs(G[r][col]) > fabs(G[piv][col])) piv = r;
+      if (fabs(G[piv][col]) < 1e-12) {
+        ok = false;
+        break;
+      }
+      if (piv != col) {
+        for (int q = 0; q < k; ++q) {
+          const double tq = G[piv][q];
+          G[piv][q] = G[col][q];
+          G[col][q] = tq;
+        }
+        const double tb = b[piv];
+        b[piv] = b[col];
+        b[col] = tb;
+      }
+      for (int r = col + 1; r < k; ++r) {
+        const double f = G[r][col] / G[col][col];
+        for (int q = col; q < k; ++q) G[r][q] -= f * G[col][q];
+        b[r] -= f * b[col];
+      }
+    }
+    if (!ok) continue;
+    for (int r = k - 1; r >= 0; --r) {
+      double a = b[r];
+      for (int q = r + 1; q < k; ++q) a -= G[r][q] * x[q];
+      x[r] = a / G[r][r];
+    }
+    for (int p = 0; p < k; ++p) xcoef[first + p] = x[p] * sc[p];
+    return;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Anderson(m): the AN2 update generalised to nh history slots,
 //   t' = Q + sum_p x_p (P_p - Q),  Q = t + beta R,
 // with the retired slot's P overwritten by Q.  Pb may alias one of the P_p;
@@ -196,7 +310,9 @@ __global__ void k_step_finish(const double *__restrict__ sums, const double *__r
 template <class Re>
 __global__ void k_andm_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
                               const V16<Re> *__restrict__ R, HistUpdateT<Re> H, V16<Re> *Pb,
-                              Re beta, size_t nv) {
+                              const double *__restrict__ xcoef, Re beta, size_t nv,
+                              const int *done) {
+  if (done && *done) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const V16<Re> ti = ld_nt(t + i), ri = ld_nt(R + i);
@@ -208,7 +324,7 @@ __global__ void k_andm_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restr
     if (h < H.nh) {
       // Pb may be the oldest P_h: this thread reads element i before it writes it
       const V16<Re> p = ld_nt(reinterpret_cast<const V16<Re> *>(H.p[h]) + i);
-      const Re x = (Re)H.x[h];
+      const Re x = (Re)xcoef[h];
 #pragma unroll
       for (int k = 0; k < V16<Re>::K; ++k) a.a[k] = fma(x, p.a[k] - q.a[k], a.a[k]);
     }
@@ -714,11 +830,27 @@ hipError_t run_an2_update(Re *tnew, const Re *t, const Re *R, const Re *P0, cons
 
 template <class Re>
 hipError_t run_andm_update(Re *tnew, const Re *t, const Re *R, const HistUpdateT<Re> &H, Re *Pb,
-                           double beta, size_t n, hipStream_t st) {
+                           const double *xcoef, double beta, size_t n, const int *done,
+                           hipStream_t st) {
   if ((n % V16<Re>::K) || H.nh < 0 || H.nh > RISM_MAX_HIST) return hipErrorInvalidValue;
   const size_t nv = nvec<Re>(n);
   hipLaunchKernelGGL(k_andm_update<Re>, dim3(nblk(nv, 256)), dim3(256), 0, st, (V16<Re> *)tnew,
-                     (const V16<Re> *)t, (const V16<Re> *)R, H, (V16<Re> *)Pb, (Re)beta, nv);
+                     (const V16<Re> *)t, (const V16<Re> *)R, H, (V16<Re> *)Pb, xcoef, (Re)beta,
+                     nv, done);
+  return hipGetLastError();
+}
+
+hipError_t run_andm_finish(const double *sums, const double *rho, int natv, size_t N, double tol,
+                           int width, double *D, int m1, int cur, const int *rb, int nh,
+                           int mixing, double *xcoef, double *scal, double *rms_hist, int *done,
+                           hipStream_t st) {
+  if (nh < 0 || nh > RISM_MAX_HIST || m1 > RISM_MAX_HIST + 1 || width < 1 + nh)
+    return hipErrorInvalidValue;
+  AndmHist H{};
+  H.nh = nh;
+  for (int p = 0; p < nh; ++p) H.rb[p] = rb[p];
+  hipLaunchKernelGGL(k_andm_finish, dim3(1), dim3(64), 0, st, sums, rho, natv, N, tol, width, D,
+                     m1, cur, H, mixing, xcoef, scal, rms_hist, done);
   return hipGetLastError();
 }
 
@@ -735,8 +867,8 @@ hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st) {
                                          Re *, const double *, double, size_t, const int *,   \
                                          hipStream_t);                                        \
   template hipError_t run_andm_update<Re>(Re *, const Re *, const Re *,                       \
-                                          const HistUpdateT<Re> &, Re *, double, size_t,      \
-                                          hipStream_t);                                       \
+                                          const HistUpdateT<Re> &, Re *, const double *,      \
+                                          double, size_t, const int *, hipStream_t);          \
   template hipError_t run_fields<Re>(const Re *, const Re *, const Re *, const Params &,      \
                                      size_t, int, double *, hipStream_t);                     \
   template hipError_t run_thermo_partial<Re>(const Re *, const Re *, const Re *, const Re *,  \

@@ -154,8 +154,11 @@ __device__ __forceinline__ void oz_points_reg(cx<Re> *lds, int CS, const int (&o
 // natv * T threads.  SHELL selects the chi source at compile time (the
 // cubic-box shell table or the spline at |k|) and NVMAX bounds the site
 // count, so each instantiation carries only the registers it needs.
-// dynamic LDS of the x passes, shared by all instantiations
-extern __shared__ double xsm[];
+// dynamic LDS of the x passes, shared by all instantiations; 16-byte
+// aligned, so FP64 complex values move as ds_read_b128 / ds_write_b128
+// (with the default 8-byte alignment they split into ds_read2_b64 pairs,
+// with about eight bank-conflict cycles per LDS instruction)
+extern __shared__ __attribute__((aligned(16))) double xsm[];
 
 template <int L, int NVMAX, bool SHELL, class Re>
 __global__ void __launch_bounds__(512)

@@ -193,6 +193,33 @@ def test_anderson_m_trajectory(rism, oracle, gpu, case24):
     g.close()
 
 
+def test_anderson_m16_tiny_tile(rism, oracle, gpu, golden):
+    """Anderson(16) on a 16^3 grid: the FP64 residual tile of nz = 16 has
+    16 threads, fewer than the 17 sums per workgroup (R.R and 16 history
+    dots), and every column must still be written.  20 steps (2 Picard, 18
+    mixing, the history full from step 18) follow the oracle's AndersonM(16)
+    to 1e-6 (its least squares is solved from the difference vectors, ours
+    from their normal equations, whose conditioning at 16 vectors is the
+    square of theirs); a missing column would put garbage into the m x m
+    solve."""
+    sol = random_walk_solute(6, seed=11, total_charge=0.0)
+    sol = oracle.Solute(sol["sig"], sol["eps"], sol["q"], sol["xyz"])
+    sol.xyz, sol.shift = oracle.centering(sol.xyz)
+    s, d = _solvent(oracle, golden)
+    grid, box = (16, 16, 16), (12.0, 12.0, 12.0)
+    o = oracle.OracleRISM(grid, box, s, sol, "KH", count=2, m=0.6, mp=0.2, tol=1e-14).setup()
+    ro = o.solve_anderson_m(16, maxstep=20)
+    g = rism.RISM3D(grid, box, d, _solute(sol), anderson_m=16)
+    g.initialize()
+    st, n, rms = g.iterate(maxstep=20, tol=1e-14)
+    t = g.fields(rism.FIELD_T).reshape(-1)
+    g.close()
+    print("rms", rms, ro.rms[-1], "t", np.abs(t - ro.t).max() / np.abs(ro.t).max())
+    assert (st, n) == (rism.STATUS_MAXSTEP, 20)
+    assert abs(rms - ro.rms[-1]) < 1e-6 * ro.rms[0]
+    assert np.abs(t - ro.t).max() < 1e-6 * np.abs(ro.t).max()
+
+
 def test_anderson_m_same_fixed_point_as_an2(rism, oracle, gpu, case24):
     """Anderson(10) and the reference's AN2 reach the same solution: solved
     to rms 1e-9, the .xmu quantities agree to 1e-6 relative."""
