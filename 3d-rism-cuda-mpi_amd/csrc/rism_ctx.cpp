@@ -13,8 +13,11 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <map>
+#include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/rism3d_c.h"
@@ -576,7 +579,7 @@ int calc_step(rism_ctx *c, double *rms, bool lin = false, double tol = -1.0) {
                               r0, c->partial, L.tb.twz, L.tb.zpre, L.tb.zpost, use_hist, c->st));
     }
   }
-  const size_t nb = rism::zinv_num_blocks<Re>(c->P, andm ? 1 : 0) / c->natv;
+  const size_t nb = rism::zinv_num_blocks<Re>(P, andm ? 1 : 0) / c->natv;
   {
     KTime kt(c, RISM_K_REDUCE);
     HIPCHECK(rism::run_reduce_cols(c->partial, nb, W, c->natv, c->sums, P.done, c->st));
@@ -1047,6 +1050,166 @@ int setup_enqueue(rism_ctx *c) {
   return RISM_OK;
 }
 
+// ---- autotuner ----------------------------------------------------------------
+// The forward z, residual z and y passes are built in up to three launch
+// plans each (tile widths, radix-4 or radix-8 stages, waves per SIMD; see
+// rism_kzpass.h and rism_kypass.hip).  Which is fastest depends on the grid
+// (the FFT length, the row width nx, the site count) and the bulk type, so
+// the plan is picked per grid shape by timing every plan the build offers on
+// the context's own buffers (one warm-up launch, then the mean of three),
+// once per process and shape (the choice is cached for the contexts that
+// follow).  RISM_TUNE=0 keeps the default plans; RISM_TUNE_FILE=<path>
+// reads earlier choices from that file and appends new ones, so a later
+// process reuses them without timing.  Plans differ in their radix orders,
+// so results can differ in the last bits between plans, never beyond FP64
+// (FP32) rounding.
+namespace tune {
+using Key = std::tuple<int, int, int, int, int, int>;  // pass, nx, ny, nz, natv, fp32
+struct Rec {
+  int choice;
+  float ms[rism::kPlans];  // < 0: plan not offered
+};
+std::mutex mu;
+std::map<Key, Rec> table;
+bool loaded = false;
+
+bool enabled() {
+  const char *e = std::getenv("RISM_TUNE");
+  return !(e && e[0] == '0');
+}
+
+// RISM_TUNE_PLANS=zf,zi,y forces these plans (where the build offers them,
+// else the default) instead of timing: reproducible plans across processes
+// and a way to check each plan.  Returns -1 when unset.
+int forced(int pass) {
+  const char *e = std::getenv("RISM_TUNE_PLANS");
+  if (!e) return -1;
+  int v[3] = {0, 0, 0};
+  if (std::sscanf(e, "%d,%d,%d", &v[0], &v[1], &v[2]) != 3) return -1;
+  return v[pass];
+}
+
+// RISM_TUNE_FILE lines: pass nx ny nz natv fp32 choice ms0 ms1 ms2
+void load_locked() {
+  if (loaded) return;
+  loaded = true;
+  const char *path = std::getenv("RISM_TUNE_FILE");
+  if (!path) return;
+  std::ifstream in(path);
+  int a[7];
+  float m[rism::kPlans];
+  while (in >> a[0] >> a[1] >> a[2] >> a[3] >> a[4] >> a[5] >> a[6] >> m[0] >> m[1] >> m[2]) {
+    if (a[6] < 0 || a[6] >= rism::kPlans) continue;
+    Rec r{a[6], {m[0], m[1], m[2]}};
+    table[Key{a[0], a[1], a[2], a[3], a[4], a[5]}] = r;
+  }
+}
+
+void save_locked(const Key &k, const Rec &r) {
+  const char *path = std::getenv("RISM_TUNE_FILE");
+  if (!path) return;
+  std::ofstream out(path, std::ios::app);
+  out << std::get<0>(k) << " " << std::get<1>(k) << " " << std::get<2>(k) << " "
+      << std::get<3>(k) << " " << std::get<4>(k) << " " << std::get<5>(k) << " " << r.choice;
+  for (float m : r.ms) out << " " << m;
+  out << "\n";
+}
+}  // namespace tune
+
+// one launch of `pass` at level Re in the plan set in c->P
+template <class Re>
+int tune_launch(rism_ctx *c, int pass) {
+  Level<Re> &L = lv<Re>(c);
+  rism::Params P = c->P;
+  P.mode = 0;
+  P.done = nullptr;
+  switch (pass) {
+    case rism::TP_ZFWD:
+      HIPCHECK(rism::run_zfwd(P, L.T[0], L.u, L.fr, L.spec, L.tb.twz, L.tb.zpre, L.tb.zpost,
+                              c->st));
+      break;
+    case rism::TP_ZINV:
+      HIPCHECK(rism::run_zinv(P, L.spec, L.T[0], L.u, L.R[0], L.R[1], L.R[2], (const Re *)nullptr,
+                              c->partial, L.tb.twz, L.tb.zpre, L.tb.zpost, 1, c->st));
+      break;
+    default:
+      HIPCHECK(rism::run_y(P, L.spec, -1, L.tb.twy, L.tb.prey, c->st));
+      break;
+  }
+  return RISM_OK;
+}
+
+// Picks the plan of each tuned pass for level Re (the cache first, else by
+// timing) and records it in c->P.  The level's fields hold no data yet:
+// they are cleared first so every plan runs on the same (zero) values.
+template <class Re>
+int tune_level(rism_ctx *c) {
+  const int f = sizeof(Re) == 4 ? 1 : 0;
+  Level<Re> &L = lv<Re>(c);
+  const size_t nn = c->N * c->natv;
+  bool cleared = false;
+  for (int pass = 0; pass < rism::TP_COUNT; ++pass) {
+    int *slot = pass == rism::TP_ZFWD ? &c->P.var_zf[f]
+                                      : (pass == rism::TP_ZINV ? &c->P.var_zi[f] : &c->P.var_y[f]);
+    *slot = 0;
+    // the residual pass of Anderson(m) contexts keeps its single plan
+    if (pass == rism::TP_ZINV && c->opts.anderson_m > 0) continue;
+    const int len = pass == rism::TP_Y ? c->ny : c->nz / 2;
+    const int fv = tune::forced(pass);
+    if (fv >= 0) {
+      *slot = (fv < rism::kPlans && rism::plan_offered(pass, len, f, fv)) ? fv : 0;
+      continue;
+    }
+    int offered = 0;
+    for (int v = 0; v < rism::kPlans; ++v) offered += rism::plan_offered(pass, len, f, v) ? 1 : 0;
+    if (offered < 2) continue;
+    const tune::Key key{pass, c->nx, c->ny, c->nz, c->natv, f};
+    {
+      std::lock_guard<std::mutex> g(tune::mu);
+      tune::load_locked();
+      auto it = tune::table.find(key);
+      if (it != tune::table.end()) {
+        *slot = it->second.choice;
+        continue;
+      }
+    }
+    if (!cleared) {
+      HIPCHECK(hipMemsetAsync(L.T[0], 0, sizeof(Re) * nn, c->st));
+      HIPCHECK(hipMemsetAsync(L.u, 0, sizeof(Re) * nn, c->st));
+      HIPCHECK(hipMemsetAsync(L.fr, 0, sizeof(Re) * c->N, c->st));
+      HIPCHECK(hipMemsetAsync(L.spec, 0, sizeof(cx<Re>) * nn / 2, c->st));
+      for (int i = 0; i < 3; ++i)
+        if (L.R[i]) HIPCHECK(hipMemsetAsync(L.R[i], 0, sizeof(Re) * nn, c->st));
+      cleared = true;
+    }
+    tune::Rec rec{0, {-1.f, -1.f, -1.f}};
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    int rc = RISM_OK;
+    for (int v = 0; v < rism::kPlans && !rc; ++v) {
+      if (!rism::plan_offered(pass, len, f, v)) continue;
+      *slot = v;
+      rc = tune_launch<Re>(c, pass);  // warm-up (first launch of the plan)
+      if (!rc && hipEventRecord(e0, c->st) != hipSuccess) rc = RISM_ERR_HIP;
+      for (int r = 0; r < 3 && !rc; ++r) rc = tune_launch<Re>(c, pass);
+      if (!rc && (hipEventRecord(e1, c->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = RISM_ERR_HIP;
+      float ms = 0.f;
+      if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
+      rec.ms[v] = ms / 3.f;
+      if (rec.ms[v] < rec.ms[rec.choice] || rec.ms[rec.choice] < 0.f) rec.choice = v;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc == RISM_ERR_HIP ? rism::set_error(rc, "autotuner launch failed") : rc;
+    *slot = rec.choice;
+    std::lock_guard<std::mutex> g(tune::mu);
+    if (tune::table.emplace(key, rec).second) tune::save_locked(key, rec);
+  }
+  return RISM_OK;
+}
+
 // Device buffers of one level: u, fr, fk, t (ping-pong) and the AN2 or
 // Anderson(m) slots; the FP32 level's spectrum lives in the FP64 one's memory
 // (the two levels never run at the same time).
@@ -1213,9 +1376,10 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
         hipMemsetAsync(c->xcoef, 0, sizeof(double) * RISM_MAX_HIST, c->st) != hipSuccess)
       return fail(rism::set_error(RISM_ERR_HIP, "clear of the Anderson(m) state failed"));
   }
-  c->zinv_blocks = std::max(std::max(rism::zinv_num_blocks<double>(c->P, 0),
+  // the partial-sum buffer holds the most workgroups any plan launches
+  c->zinv_blocks = std::max(std::max(rism::zinv_num_blocks<double>(c->P, -1),
                                      rism::zinv_num_blocks<double>(c->P, 1)),
-                            std::max(rism::zinv_num_blocks<float>(c->P, 0),
+                            std::max(rism::zinv_num_blocks<float>(c->P, -1),
                                      rism::zinv_num_blocks<float>(c->P, 1)));
   if ((rc = c->alloc(&c->partial, c->zinv_blocks * kSumWidth))) return fail(rc);
   const int tb = rism::thermo_blocks(c->N);
@@ -1228,6 +1392,10 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   if (hipHostMalloc((void **)&c->hscal, 8 * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&c->hflag, 3 * sizeof(int)) != hipSuccess)
     return fail(rism::set_error(RISM_ERR_NOMEM, "hipHostMalloc failed"));
+  if (tune::enabled() || tune::forced(0) >= 0) {
+    if ((rc = tune_level<double>(c))) return fail(rc);
+    if (c->opts.precision == 1 && (rc = tune_level<float>(c))) return fail(rc);
+  }
   reset_an2(c);
   if (hipMemsetAsync(c->d.T[0], 0, sizeof(double) * nn, c->st) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess)
@@ -1906,6 +2074,27 @@ int rism_k_copy_bench(size_t bytes, int reps, double *avg_ms) {
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
   return rc;
+}
+
+int rism_tune_report(rism_tune_entry *out, int cap) {
+  std::lock_guard<std::mutex> g(tune::mu);
+  tune::load_locked();
+  int n = 0;
+  for (const auto &kv : tune::table) {
+    if (out && n < cap) {
+      rism_tune_entry &e = out[n];
+      e.pass = std::get<0>(kv.first);
+      e.nx = std::get<1>(kv.first);
+      e.ny = std::get<2>(kv.first);
+      e.nz = std::get<3>(kv.first);
+      e.natv = std::get<4>(kv.first);
+      e.fp32 = std::get<5>(kv.first);
+      e.choice = kv.second.choice;
+      for (int v = 0; v < 3; ++v) e.ms[v] = kv.second.ms[v];
+    }
+    ++n;
+  }
+  return n;
 }
 
 int rism_k_an2_state(rism_ctx *c, double *x, double *s, int *count, int *binary) {

@@ -33,7 +33,18 @@ struct Params {
   // device flag of a solve driven without host synchronisation: once it is
   // non-zero (converged or diverged) every step kernel returns at entry
   const int *done;
+  // launch plans of the forward z, residual z and y passes picked by the
+  // autotuner, per bulk type ([0] FP64, [1] FP32); 0 is the default plan
+  int var_zf[2], var_zi[2], var_y[2];
 };
+
+// Launch plans the autotuner chooses between (rism_ctx.cpp): whether plan
+// v (0, 1, 2) of a pass is built for FFT length L (nz/2 for the z passes,
+// ny for the y passes) and the bulk type; plan 0 always is.
+enum TunedPass { TP_ZFWD = 0, TP_ZINV = 1, TP_Y = 2, TP_COUNT = 3 };
+constexpr int kPlans = 3;
+bool plan_offered(int pass, int L, int fp32, int v);
+bool y_plan_offered(int L, int fp32, int v);
 
 // History of the Anderson(m) accelerator passed to kernels by value.
 #define RISM_MAX_HIST 16

@@ -347,24 +347,78 @@ __global__ void __launch_bounds__(B *TZ)
 static inline unsigned zgrid(const Params &P, int B) {
   return (unsigned)(((P.nx + B - 1) / B) * P.ny * P.natv);
 }
-template <int M, int KIND, class Re>
+
+// Launch plans of the z passes, chosen per grid by the autotuner (plan 0 is
+// the ZTile default).  Forward pass: 0 the ZTile tile (FP64 columns of
+// M >= 128 in radix-4 stages, 16 columns, two workgroups per CU); 1 the wide
+// radix-8 tile (ZTile::B columns, one workgroup per CU when it needs the
+// LDS); 2 half as many columns in radix 8, compiled for two workgroups per
+// CU.  Residual pass: 0 the ZTile tile; 1 half as many columns, compiled for
+// twice the waves.  A plan narrower than 64 bytes of contiguous x per row
+// is not offered.
+template <int M, class Re, int V>
+struct ZfPlan {
+  using Z = ZTile<M, Re>;
+  static constexpr int B = V == 0 ? Z::BF : (V == 1 ? Z::B : (Z::B > 1 ? Z::B / 2 : 1));
+  static constexpr int T = V == 0 ? Z::TF : Z::T;
+  static constexpr int MAXR = V == 0 ? Z::MAXRF : 8;
+  static constexpr int WPE = V == 0 ? (Z::MAXRF == 4 ? 8 : 1) : (V == 1 ? 1 : 2);
+};
+template <int M, class Re, int V>
+constexpr bool zf_plan_offered() {
+  if constexpr (V == 0) {
+    return true;
+  } else {
+    using A = ZfPlan<M, Re, 0>;
+    using P1 = ZfPlan<M, Re, V>;
+    if (P1::B * (int)sizeof(Re) < 64) return false;
+    const bool same0 = A::B == P1::B && A::T == P1::T && A::MAXR == P1::MAXR;
+    if constexpr (V == 1) {
+      return !same0;
+    } else {
+      using Q = ZfPlan<M, Re, 1>;
+      const bool same1 = Q::B == P1::B && Q::T == P1::T && Q::MAXR == P1::MAXR;
+      return !same0 && !same1;
+    }
+  }
+}
+template <int M, class Re, int HM, int V>
+struct ZiPlan {
+  static constexpr int B0 = HM ? ZTile<M, Re>::BH : ZTile<M, Re>::B;
+  static constexpr int B = V == 0 ? B0 : (B0 > 1 ? B0 / 2 : 1);
+  static constexpr int T = ZTile<M, Re>::T;
+  static constexpr int W0 = zinv_waves_per_eu<HM, Re>();
+  static constexpr int WPE = V == 0 ? W0 : (2 * W0 > 8 ? 8 : 2 * W0);
+};
+template <int M, class Re, int V>
+constexpr bool zi_plan_offered() {
+  return V == 0 || (ZiPlan<M, Re, 0, V>::B * (int)sizeof(Re) >= 64 &&
+                    ZiPlan<M, Re, 0, V>::B != ZiPlan<M, Re, 0, 0>::B);
+}
+
+template <int M, int KIND, class Re, int V = 0>
 static void launch_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<Re> *spec,
                         const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost,
                         hipStream_t st) {
-  using Z = ZTile<M, Re>;
-  constexpr int B = Z::BF, TZ = Z::TF;
-  hipLaunchKernelGGL((k_zfwd_closure<M, B, TZ, Z::MAXRF, KIND, Re>), dim3(zgrid(P, B)),
-                     dim3(B * TZ), 0, st, t, u, fr, P, spec, twM, zpre, zpost);
+  using Z = ZfPlan<M, Re, V>;
+  hipLaunchKernelGGL((k_zfwd_closure<M, Z::B, Z::T, Z::MAXR, KIND, Re, Z::WPE>),
+                     dim3(zgrid(P, Z::B)), dim3(Z::B * Z::T), 0, st, t, u, fr, P, spec, twM,
+                     zpre, zpost);
 }
-template <int M, int HM, int KIND, class Re>
+template <int M, int HM, int KIND, class Re, int V = 0>
 static void launch_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
                         const Re *R0, const Re *R1, const Re *r0, const HistT<Re> &H,
                         double *partial, const cx<Re> *twM, const cx<Re> *zpre,
                         const cx<Re> *zpost, hipStream_t st) {
-  constexpr int B = HM ? ZTile<M, Re>::BH : ZTile<M, Re>::B, TZ = ZTile<M, Re>::T;
-  hipLaunchKernelGGL((k_zinv_resid<M, B, TZ, HM, KIND, Re>), dim3(zgrid(P, B)), dim3(B * TZ), 0,
-                     st, spec, t, u, R, R0, R1, r0, H, P, partial, twM, zpre, zpost);
+  using Z = ZiPlan<M, Re, HM, V>;
+  hipLaunchKernelGGL((k_zinv_resid<M, Z::B, Z::T, HM, KIND, Re, Z::WPE>), dim3(zgrid(P, Z::B)),
+                     dim3(Z::B * Z::T), 0, st, spec, t, u, R, R0, R1, r0, H, P, partial, twM,
+                     zpre, zpost);
 }
+
+// Lengths with more than one plan: the autotuner's search space (smaller
+// columns keep their single plan).
+#define RISM_TUNED(n) ((n) >= 48)
 
 // The kinds each bulk type is built for: FP64 runs solver steps, Picard
 // steps and the plain transforms of the kernel-level entry points; FP32 (the
@@ -374,6 +428,21 @@ template <class Re>
 constexpr bool zkind_built(int kind) {
   return kind == ZK_STEP || kind == ZK_PICARD ||
          (sizeof(Re) == 8 ? kind == ZK_PLAIN : kind == ZK_DIFF);
+}
+
+// solver steps of the forward pass in the tuned plan
+template <int M, class Re>
+static void zfwd_step_plan(const Params &P, const Re *t, const Re *u, const Re *fr, cx<Re> *spec,
+                           const cx<Re> *twM, const cx<Re> *zpre, const cx<Re> *zpost,
+                           hipStream_t st) {
+  const int v = P.var_zf[sizeof(Re) == 4];
+  if constexpr (RISM_TUNED(M) && zf_plan_offered<M, Re, 1>()) {
+    if (v == 1) return launch_zfwd<M, ZK_STEP, Re, 1>(P, t, u, fr, spec, twM, zpre, zpost, st);
+  }
+  if constexpr (RISM_TUNED(M) && zf_plan_offered<M, Re, 2>()) {
+    if (v == 2) return launch_zfwd<M, ZK_STEP, Re, 2>(P, t, u, fr, spec, twM, zpre, zpost, st);
+  }
+  launch_zfwd<M, ZK_STEP, Re>(P, t, u, fr, spec, twM, zpre, zpost, st);
 }
 
 template <class Re>
@@ -386,7 +455,7 @@ hipError_t zfwd_dispatch(const Params &P, const Re *t, const Re *u, const Re *fr
 #define X(n)                                                                      \
   case n:                                                                         \
     if (kind == ZK_STEP)                                                          \
-      launch_zfwd<n, ZK_STEP>(P, t, u, fr, spec, twM, zpre, zpost, st);           \
+      zfwd_step_plan<n, Re>(P, t, u, fr, spec, twM, zpre, zpost, st);             \
     else if constexpr (sizeof(Re) == 8)                                           \
       launch_zfwd<n, ZK_PLAIN>(P, t, u, fr, spec, twM, zpre, zpost, st);          \
     else                                                                          \
@@ -398,6 +467,20 @@ hipError_t zfwd_dispatch(const Params &P, const Re *t, const Re *u, const Re *fr
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// AN2 and Picard steps of the residual pass in the tuned plan
+template <int M, int KIND, class Re>
+static void zinv_step_plan(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
+                           const Re *R0, const Re *R1, const Re *r0, const HistT<Re> &H,
+                           double *partial, const cx<Re> *twM, const cx<Re> *zpre,
+                           const cx<Re> *zpost, hipStream_t st) {
+  if constexpr (RISM_TUNED(M) && zi_plan_offered<M, Re, 1>()) {
+    if (P.var_zi[sizeof(Re) == 4] == 1)
+      return launch_zinv<M, 0, KIND, Re, 1>(P, spec, t, u, R, R0, R1, r0, H, partial, twM,
+                                            zpre, zpost, st);
+  }
+  launch_zinv<M, 0, KIND, Re>(P, spec, t, u, R, R0, R1, r0, H, partial, twM, zpre, zpost, st);
 }
 
 template <class Re>
@@ -414,11 +497,11 @@ hipError_t zinv_dispatch(const Params &P, const cx<Re> *spec, const Re *t, const
 #define X(n)                                                                              \
   case n:                                                                                 \
     if (kind == ZK_STEP)                                                                  \
-      launch_zinv<n, 0, ZK_STEP>(P, spec, t, u, R, R0, R1, r0, H, partial, twM, zpre,     \
-                                 zpost, st);                                              \
+      zinv_step_plan<n, ZK_STEP, Re>(P, spec, t, u, R, R0, R1, r0, H, partial, twM, zpre, \
+                                     zpost, st);                                          \
     else if (kind == ZK_PICARD)                                                           \
-      launch_zinv<n, 0, ZK_PICARD>(P, spec, t, u, R, R0, R1, r0, H, partial, twM, zpre,   \
-                                   zpost, st);                                            \
+      zinv_step_plan<n, ZK_PICARD, Re>(P, spec, t, u, R, R0, R1, r0, H, partial, twM,     \
+                                       zpre, zpost, st);                                  \
     else if constexpr (sizeof(Re) == 8)                                                   \
       launch_zinv<n, 0, ZK_PLAIN>(P, spec, t, u, R, R0, R1, r0, H, partial, twM, zpre,    \
                                   zpost, st);                                             \
@@ -466,16 +549,21 @@ hipError_t zinv_hist_dispatch(const Params &P, const cx<Re> *spec, const Re *t, 
   return hipGetLastError();
 }
 
-// Workgroups of one z pass; the same for both bulk types (the FP32 tiles are
-// twice as wide, so pass the type the solver runs in).
+// Workgroups of the residual pass in the plan it runs with (the tuned plan
+// for AN2 and Picard steps, the default for the Anderson(m) sums, the
+// difference form and the plain inverse); plan < 0: the most workgroups any
+// plan of this type launches (the size of the partial-sum buffer).
 template <class Re>
-size_t zinv_blocks_of(const Params &P, int hm) {
+size_t zinv_blocks_of(const Params &P, int hm, int plan) {
   const int M = P.nz / 2;
   int B = 16;
+  const int v = plan >= 0 ? plan : 1;
   switch (M) {
-#define X(n)                                         \
-  case n:                                            \
-    B = hm ? ZTile<n, Re>::BH : ZTile<n, Re>::B;     \
+#define X(n)                                                                         \
+  case n:                                                                            \
+    B = hm ? ZTile<n, Re>::BH : ZTile<n, Re>::B;                                     \
+    if constexpr (RISM_TUNED(n) && zi_plan_offered<n, Re, 1>())                      \
+      if (!hm && v == 1) B = ZiPlan<n, Re, 0, 1>::B;                                 \
     break;
     RISM_LENGTHS(X)
 #undef X
@@ -489,7 +577,9 @@ size_t zinv_blocks_of(const Params &P, int hm) {
 // its own translation unit.
 template <class Re>
 size_t zinv_num_blocks(const Params &P, int hm) {
-  return zinv_blocks_of<Re>(P, hm);
+  if (hm < 0) return zinv_blocks_of<Re>(P, 0, -1);  // largest over the plans
+  const bool tuned = !hm && (P.mode == 0);
+  return zinv_blocks_of<Re>(P, hm, tuned ? P.var_zi[sizeof(Re) == 4] : 0);
 }
 template <class Re>
 hipError_t run_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr, cx<Re> *spec,

@@ -43,7 +43,7 @@ EXPORTS = [
     "rism_k_fft_c2r", "rism_k_get_fk", "rism_k_an2_state", "rism_set_kernel_timing",
     "rism_get_kernel_stats", "rism_set_precision", "rism_get_precision",
     "rism_solve_batch", "rism_run_batch", "rism_write_huv", "rism_write_cuv",
-    "rism_analysis", "rism_k_copy_bench",
+    "rism_analysis", "rism_k_copy_bench", "rism_tune_report",
 ]
 KERNELS = ["zfwd_closure", "ypass_fwd", "xpass_oz", "ypass_inv", "zinv_resid", "reduce",
            "an2_update", "potential_r", "fk_half", "thermo_partial"]
@@ -109,6 +109,12 @@ class Timing(ctypes.Structure):
 
 class KStat(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_long), ("total_ms", ctypes.c_double)]
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("pass_", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("natv", ctypes.c_int), ("fp32", ctypes.c_int),
+                ("choice", ctypes.c_int), ("ms", ctypes.c_float * 3)]
 
 
 class InputC(ctypes.Structure):
@@ -178,6 +184,8 @@ def lib():
     L.rism_set_kernel_timing.argtypes = [vp, ci]
     L.rism_get_kernel_stats.argtypes = [vp, ctypes.POINTER(KStat)]
     L.rism_k_copy_bench.argtypes = [ctypes.c_size_t, ci, _dp]
+    L.rism_tune_report.argtypes = [ctypes.c_void_p, ci]
+    L.rism_tune_report.restype = ci
     _lib = L
     return L
 
@@ -280,6 +288,24 @@ def copy_bench(nbytes, reps=5):
     ms = ctypes.c_double(0.0)
     _check(lib().rism_k_copy_bench(int(nbytes), int(reps), ctypes.byref(ms)))
     return ms.value
+
+
+TUNED_PASSES = ("zfwd_closure", "zinv_resid", "ypass")
+
+
+def tune_report():
+    """The autotuner's choices so far in this process (rism_tune_report):
+    one dict per tuned pass and grid shape with the chosen plan and the mean
+    launch time of each plan (None: not built for that length)."""
+    n = lib().rism_tune_report(None, 0)
+    arr = (TuneEntry * max(n, 1))()
+    n = lib().rism_tune_report(arr, n)
+    out = []
+    for e in arr[:n]:
+        out.append({"pass": TUNED_PASSES[e.pass_], "grid": [e.nx, e.ny, e.nz], "natv": e.natv,
+                    "dtype": "f32" if e.fp32 else "f64", "choice": e.choice,
+                    "ms": [m if m >= 0 else None for m in e.ms]})
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -286,6 +286,7 @@ def phase_headline(args, budget, R):
     })
     log(f"fp64: {s_per:.3f} s/solve over {nt} solves ({iters} iterations, "
         f"{iter_ms:.2f} ms/iteration)")
+    R["out"]["config"]["autotune"] = tune_summary(rism, grid)
     ctx = {"rism": rism, "grid": grid, "box": box, "solv": solv, "sol": sol, "xvk": xvk,
            "natv": natv, "N": N, "clos": clos, "am": am, "th": th, "tm": tm, "iters": iters}
     # h of the FP64 solve for rms(dh) of the mixed one, and one instrumented
@@ -314,6 +315,20 @@ def phase_headline(args, budget, R):
         R["out"]["config"]["potential_pairs_per_s"] = N * natu / (pot_ms / 1e3)
     ctx["dom"] = dom
     return ctx
+
+
+def tune_summary(rism, grid=None):
+    """The autotuner's plan per tuned pass (of this grid, or all grids):
+    mean launch ms of each plan and the gain of the chosen plan over the
+    built-in default (plan 0)."""
+    out = []
+    for e in rism.tune_report():
+        if grid is not None and e["grid"] != list(grid):
+            continue
+        ms = e["ms"]
+        out.append({"pass": e["pass"], "grid": e["grid"], "dtype": e["dtype"], "plan": e["choice"],
+                    "plan_ms": ms, "gain_vs_default": ms[0] / ms[e["choice"]]})
+    return out
 
 
 def phase_mixed(args, budget, R, ctx):

@@ -336,6 +336,23 @@ int rism_k_an2_state(rism_ctx *ctx, double *x, double *s, int *count, int *binar
  * bandwidth.  Allocates and frees its buffers. */
 int rism_k_copy_bench(size_t bytes, int reps, double *avg_ms);
 
+/* Autotuner report (no counterpart in the reference, whose transforms are
+ * cuFFT plans, fft3d.cu:61-63).  The forward z, residual z and y passes are
+ * built in up to three launch plans; each context creation picks, per grid
+ * shape and bulk type, the fastest plan by timing them on its buffers (once
+ * per process and shape; RISM_TUNE=0 keeps the default plans,
+ * RISM_TUNE_FILE=<path> persists the choices, RISM_TUNE_PLANS=zf,zi,y forces
+ * plans without timing where they are built).  Writes up to cap entries
+ * (plan 0 = the built-in default, ms[v] < 0 = plan v not built for that
+ * length) and returns how many there are. */
+typedef struct {
+  int pass; /* 0 forward z, 1 residual z, 2 y */
+  int nx, ny, nz, natv, fp32;
+  int choice;
+  float ms[3]; /* mean launch time of each plan */
+} rism_tune_entry;
+int rism_tune_report(rism_tune_entry *out, int cap);
+
 #ifdef __cplusplus
 }
 #endif
