@@ -162,6 +162,10 @@ template <class Re>
 hipError_t run_thermo_partial(const Re *t, const Re *th, const Re *u, const Re *R,
                               const Params &P, size_t N, double *partial, hipStream_t st);
 hipError_t run_apply_phase(cplx *s, const Params &P, int inverse, hipStream_t st);
+// twiddle and phase tables of the axis passes, computed on the device into
+// tw[2 nx + 2 ny + 3 nz/2]: roots(nx), roots(ny), roots(nz/2), pre(nx),
+// pre(ny), zpre, zpost (the order of a context's host-computed tables)
+hipError_t run_fft_tables(int nx, int ny, int nz, cplx *tw, hipStream_t st);
 // per-atom integrals of g (kind 0 grad, 1 euv, 2 ad1, 3 ad2); adata: per atom
 // x y z q sigma_uv[natv] eps_uv[natv]; partial: [slab][natu][width] with
 // width 3, natv, 1, 1 and atom_sum_slab(N) points per slab

@@ -982,6 +982,49 @@ hipError_t run_atom_sums(int kind, const double *t, const double *u, const Param
   return hipGetLastError();
 }
 
+// The twiddle and phase tables of the three axis passes computed on the
+// device (the host computes the same values for a context): consecutive
+// arrays roots(nx), roots(ny), roots(M), pre(nx), pre(ny), zpre(M),
+// zpost(M), M = nz/2, with roots(L)_q = e^{-2 pi i q/L}, pre(n)_j =
+// (-1)^j e^{-i pi j/n}, zpre_p = e^{-i pi p/M}, zpost_p = e^{-i pi (2p+1)/nz}.
+__global__ void k_fft_tables(int nx, int ny, int nz, cplx *__restrict__ tw) {
+  const int M = nz / 2;
+  const int total = 2 * nx + 2 * ny + 3 * M;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int k = i;
+  double a;  // angle in units of pi
+  double sg = 1.0;
+  if (k < nx) {
+    a = -2.0 * k / nx;
+  } else if ((k -= nx) < ny) {
+    a = -2.0 * k / ny;
+  } else if ((k -= ny) < M) {
+    a = -2.0 * k / M;
+  } else if ((k -= M) < nx) {
+    a = -(double)k / nx;
+    sg = (k & 1) ? -1.0 : 1.0;
+  } else if ((k -= nx) < ny) {
+    a = -(double)k / ny;
+    sg = (k & 1) ? -1.0 : 1.0;
+  } else if ((k -= ny) < M) {
+    a = -(double)k / M;
+  } else {
+    k -= M;
+    a = -(2.0 * k + 1.0) / nz;
+  }
+  double sn, cs;
+  sincospi(a, &sn, &cs);
+  tw[i] = cplx{sg * cs, sg * sn};
+}
+
+hipError_t run_fft_tables(int nx, int ny, int nz, cplx *tw, hipStream_t st) {
+  const int total = 2 * nx + 2 * ny + 3 * (nz / 2);
+  hipLaunchKernelGGL(k_fft_tables, dim3(nblk((size_t)total, 256)), dim3(256), 0, st, nx, ny, nz,
+                     tw);
+  return hipGetLastError();
+}
+
 hipError_t run_apply_phase(cplx *s, const Params &P, int inverse, hipStream_t st) {
   const size_t n = (size_t)(P.nz / 2) * P.ny * P.nx;
   hipLaunchKernelGGL(k_apply_phase, dim3(nblk(n, 256)), dim3(256), 0, st, s, P, inverse);

@@ -43,7 +43,7 @@ EXPORTS = [
     "rism_k_fft_c2r", "rism_k_get_fk", "rism_k_an2_state", "rism_set_kernel_timing",
     "rism_get_kernel_stats", "rism_set_precision", "rism_get_precision",
     "rism_solve_batch", "rism_run_batch", "rism_write_huv", "rism_write_cuv",
-    "rism_analysis", "rism_k_copy_bench", "rism_tune_report",
+    "rism_analysis", "rism_k_copy_bench", "rism_tune_report", "rism_k_fft_noalloc",
 ]
 KERNELS = ["zfwd_closure", "ypass_fwd", "xpass_oz", "ypass_inv", "zinv_resid", "reduce",
            "an2_update", "potential_r", "fk_half", "thermo_partial"]
@@ -186,6 +186,8 @@ def lib():
     L.rism_k_copy_bench.argtypes = [ctypes.c_size_t, ci, _dp]
     L.rism_tune_report.argtypes = [ctypes.c_void_p, ci]
     L.rism_tune_report.restype = ci
+    L.rism_k_fft_noalloc.argtypes = [ci, ci, ci, ci, _dp, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
     _lib = L
     return L
 
@@ -281,6 +283,16 @@ def fft_c2r(F, box):
     b = _f64(box)
     _check(lib().rism_k_fft_c2r(nx, ny, nz, _ptr(b), F.ctypes.data_as(_dp), _ptr(out)))
     return out
+
+
+def fft_noalloc(direction, grid, box, real_ptr, spec_ptr, stream=0):
+    """rism_k_fft_noalloc on device buffers given by address (e.g. a torch
+    tensor's data_ptr()): direction -1 real -> half spectrum, +1 half
+    spectrum (overwritten) -> real; enqueued on `stream`, nothing allocated."""
+    nx, ny, nz = (int(g) for g in grid)
+    b = _f64(box)
+    _check(lib().rism_k_fft_noalloc(int(direction), nx, ny, nz, _ptr(b), ctypes.c_void_p(real_ptr),
+                                    ctypes.c_void_p(spec_ptr), ctypes.c_void_p(stream)))
 
 
 def copy_bench(nbytes, reps=5):

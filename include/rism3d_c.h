@@ -353,6 +353,22 @@ typedef struct {
 } rism_tune_entry;
 int rism_tune_report(rism_tune_entry *out, int cap);
 
+/* The transform of rism_k_fft_r2c (dir = -1) or rism_k_fft_c2r (dir = +1)
+ * on caller-owned DEVICE buffers, enqueued on `stream` (a hipStream_t, NULL
+ * for the default stream), allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight loops, harnesses that forbid hipMalloc).  real: nx*ny*nz
+ * doubles [z][y][x]; spec: the k_z > 0 half spectrum, nx*ny*nz/2 complex
+ * (as rism_k_fft_r2c writes it).  dir = -1 reads real and writes spec;
+ * dir = +1 reads spec, uses it as workspace (it is overwritten) and writes
+ * real.  The tables and partial sums live in a static device array of
+ * 2 MiB per GPU (kept until the process ends): grids whose partial sums
+ * exceed 1.875 MiB (beyond 512^3) are refused with RISM_ERR_ARG.  Calls
+ * that overlap on one GPU share that array and race.  Returns once the
+ * work is enqueued. */
+int rism_k_fft_noalloc(int dir, int nx, int ny, int nz, const double box[3], double *real,
+                       double *spec, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
