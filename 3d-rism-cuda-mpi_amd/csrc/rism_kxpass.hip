@@ -180,13 +180,17 @@ __global__ void __launch_bounds__(512)
   C *row = spec + (size_t)v * rowstride + rowoff;
   C *mylds = lds + v * CS;
   // the spectrum row is read once and written once: non-temporal accesses
-  // leave the caches to the chi table, fk and the twiddles
+  // leave the caches to the chi table, fk and the twiddles.  The element
+  // indices of the first and last FFT stages are j + c with c the same for
+  // every thread, and the pre-twiddle is an exponential, so prex[j + c] =
+  // prex[j] prex[c]: one per-thread load, the rest uniform (scalar) loads.
+  const C prej = prex[j];
   auto load = [&](int i) -> C {
     const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
-    return cmul(a, prex[i]);
+    return cmul(a, cmul(prej, prex[i - j]));
   };
   auto store = [&](int i, C vv) {
-    const C o = cmulc(vv, prex[i]);
+    const C o = cmulc(vv, cmul(prej, prex[i - j]));
     __builtin_nontemporal_store(o.x, &row[i].x);
     __builtin_nontemporal_store(o.y, &row[i].y);
   };

@@ -23,15 +23,18 @@ __global__ void __launch_bounds__(B *FftShape<L>::T)
   C *col = spec + (size_t)blockIdx.y * L * nx + (ok ? x : 0);
   // the spectrum is read once and written once per pass: non-temporal
   // accesses leave the caches to the twiddle tables
+  // pre[j + c] = pre[j] pre[c] (an exponential; c is the same for every
+  // thread in the first and last stages): one per-thread load
+  const C prej = pre[j];
   auto load = [&](int i) -> C {
     if (!ok) return C{0, 0};
     const C *q = col + (size_t)i * nx;
     const C vv{__builtin_nontemporal_load(&q->x), __builtin_nontemporal_load(&q->y)};
-    return (S < 0) ? cmul(vv, pre[i]) : vv;
+    return (S < 0) ? cmul(vv, cmul(prej, pre[i - j])) : vv;
   };
   auto store = [&](int i, C vv) {
     if (!ok) return;
-    const C o = (S < 0) ? vv : cmulc(vv, pre[i]);
+    const C o = (S < 0) ? vv : cmulc(vv, cmul(prej, pre[i - j]));
     C *q = col + (size_t)i * nx;
     __builtin_nontemporal_store(o.x, &q->x);
     __builtin_nontemporal_store(o.y, &q->y);

@@ -81,21 +81,27 @@ __global__ void __launch_bounds__(B *TZ)
       return (g - (Re)1 - tt + q * fr[i]) * dv;  // trm1mt, times the forward dv
     }
   };
+  // zpre[p] = e^{-i pi p/M} is an exponential and the first stage reads
+  // elements p = j + c with c the same for every thread: zpre[p] = zpre[j]
+  // zpre[c], one per-thread load and uniform (scalar) loads
+  const C zprej = zpre[j];
   auto load = [&](int p) -> C {
     if (!ok) return C{0, 0};
-    return cmul(C{cval(2 * p), cval(2 * p + 1)}, zpre[p]);
+    return cmul(C{cval(2 * p), cval(2 * p + 1)}, cmul(zprej, zpre[p - j]));
   };
   auto store = [&](int, C) {};
   fft_column<M, -1, true, false, TZ, MAXR>(col, j, twM, load, store);
 
   C *out = spec + ((size_t)v * M * P.ny + y) * P.nx + x;
   const size_t mstride = (size_t)P.ny * P.nx;
+  // zpost[m] = e^{-i pi (2m+1)/nz} = zpost[j] zpre[m - j]
+  const C zpostj = zpost[j];
   for (int m = j; m < M; m += T) {
     const C a = col[lpad(m)];
     const C b = conj(col[lpad(M - 1 - m)]);
     const C E = (Re)0.5 * (a + b);
     const C O = mul_si<-1>((Re)0.5 * (a - b));
-    if (ok) out[(size_t)m * mstride] = E + cmul(zpost[m], O);
+    if (ok) out[(size_t)m * mstride] = E + cmul(cmul(zpostj, zpre[m - j]), O);
   }
 }
 
@@ -161,14 +167,19 @@ __global__ void __launch_bounds__(B *TZ)
   // G(m) -> Z(m) = E(m) + i O(m) as the column is read: one thread loads the
   // pair (m, M-1-m) and writes both Z values to LDS (one LDS pass and one
   // barrier fewer than staging G first: 6.6 -> 5.7 ms at 512^3, 4 sites)
+  // zpost[m] = zpost[j] zpre[m - j] and zpost[M-1-m] = -conj(zpost[m])
+  // (e^{-i pi (2m+1)/nz}, nz = 2M)
+  const C zpostj = zpost[j];
   for (int m = j; 2 * m <= M - 1; m += T) {
     const int mm = M - 1 - m;
     const C a = ok ? in[(size_t)m * plane] : C{0, 0};
     const C b = ok ? in[(size_t)mm * plane] : C{0, 0};
+    const C wm = cmul(zpostj, zpre[m - j]);
+    const C wmm{-wm.x, wm.y};
     const C Em = (Re)0.5 * (a + conj(b));
-    const C Om = cmulc((Re)0.5 * (a - conj(b)), zpost[m]);
+    const C Om = cmulc((Re)0.5 * (a - conj(b)), wm);
     const C Emm = (Re)0.5 * (b + conj(a));
-    const C Omm = cmulc((Re)0.5 * (b - conj(a)), zpost[mm]);
+    const C Omm = cmulc((Re)0.5 * (b - conj(a)), wmm);
     col[lpad(m)] = Em + mul_si<1>(Om);
     if (mm != m) col[lpad(mm)] = Emm + mul_si<1>(Omm);
   }
@@ -239,6 +250,8 @@ __global__ void __launch_bounds__(B *TZ)
       const Re *tv, *uv, *R0v, *R1v, *r0v;
       Re *Rv;
       const C *zpre;
+      C zprej;
+      int j;
       size_t base, plane;
       Re scale;
       int clos;
@@ -291,19 +304,22 @@ __global__ void __launch_bounds__(B *TZ)
       }
       __device__ __forceinline__ void put(int p, C zv, const aux_t &a) {
         if (!ok) return;
-        const C zz = scale * cmulc(zv, zpre[p]);
+        // zpre[p] = zpre[j] zpre[p - j], p - j the same for every thread
+        const C zz = scale * cmulc(zv, cmul(zprej, zpre[p - j]));
         const size_t i0 = base + (size_t)(2 * p) * plane;
         one(i0, zz.x, a.t0, a.u0, a.a0, a.b0, a.c0);
         one(i0 + plane, zz.y, a.t1, a.u1, a.a1, a.b1, a.c1);
       }
-    } store{tv, uv, R0v, R1v, r0 ? r0 + voff : nullptr, Rv, zpre, base, plane, scale, clos, ok, {}};
+    } store{tv,   uv,   R0v,  R1v,   r0 ? r0 + voff : nullptr, Rv, zpre, zpre[j], j, base,
+            plane, scale, clos, ok, {}};
     fft_column<M, +1, false, true, TZ, MAXRI>(col, j, twM, load, store);
 #pragma unroll
     for (int k = 0; k < W; ++k) s[k] = store.s[k];
   } else {
+    const C zprej = zpre[j];
     auto store = [&](int p, C zv) {
       if (!ok) return;
-      const C zz = scale * cmulc(zv, zpre[p]);
+      const C zz = scale * cmulc(zv, cmul(zprej, zpre[p - j]));
       resid(2 * p, zz.x);
       resid(2 * p + 1, zz.y);
     };
