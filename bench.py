@@ -530,18 +530,22 @@ def kernel_key(name):
 
 
 def read_counters(outdir, counter):
-    """{kernel key: [value per dispatch]} from rocprofv3 csv output."""
+    """{kernel key: [value per dispatch]} from rocprofv3 csv output, for each
+    key the instantiation with the most dispatches: the solver step's kernel
+    in the chosen launch plan (the autotuner's trials of the other plans and
+    the Picard-step variants run a few times each)."""
     files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
-    vals = {}
+    per = {}
     for f in files:
         with open(f, newline="") as fh:
             for row in csv.DictReader(fh):
                 if row.get("Counter_Name") != counter:
                     continue
-                k = kernel_key(row.get("Kernel_Name", ""))
+                name = row.get("Kernel_Name", "")
+                k = kernel_key(name)
                 if k:
-                    vals.setdefault(k, []).append(float(row["Counter_Value"]))
-    return vals
+                    per.setdefault(k, {}).setdefault(name, []).append(float(row["Counter_Value"]))
+    return {k: max(d.values(), key=len) for k, d in per.items()}
 
 
 def phase_traffic(args, budget, R, ctx):
@@ -575,7 +579,7 @@ def phase_traffic(args, budget, R, ctx):
         if k == "copy" or k not in res["WRITE_SIZE"]:
             continue
         f, w = res["FETCH_SIZE"][k], res["WRITE_SIZE"][k]
-        # the last launches: the mixing steps (the first two are Picard steps)
+        # the last launches: the solver's steps (the first are the autotuner's)
         f, w = f[len(f) // 2:], w[len(w) // 2:]
         rd = float(np.mean(f)) * 1024.0 * fcal
         wr = float(np.mean(w)) * 1024.0 * wcal
