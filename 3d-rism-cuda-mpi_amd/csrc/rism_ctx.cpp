@@ -740,7 +740,8 @@ int rebase_up(rism_ctx *c, int k0) {
 // *fallback set and the state at the FP64 level, and the caller continues
 // with plain FP64 steps from there.  A non-finite residual in phase 1 (the
 // FP32 exponent overflows above 88, FP64 only above 709) sets *restart: the
-// caller solves in FP64 from the initial t, which phase 1 left untouched.
+// caller solves in FP64 from the t phase 1 started from (its FP64 copy,
+// which phase 1 leaves untouched).
 int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, bool &conver,
                 int &low_steps, bool &fallback, bool &restart) {
   const bool verbose = c->opts.verbose != 0;
@@ -757,7 +758,7 @@ int solve_mixed(rism_ctx *c, int maxstep, double tol, int &istep, double &rms, b
     ++low_steps;
     if (!std::isfinite(rms)) {
       if (verbose)
-        std::cout << " Step = " << istep << " FP32 overflow: FP64 from the initial Tuv"
+        std::cout << " Step = " << istep << " FP32 overflow: FP64 from the starting Tuv"
                   << std::endl;
       restart = true;
       return RISM_OK;
@@ -1468,13 +1469,20 @@ int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, do
   double rms = 0.0;
   if (c->opts.precision == 1 && c->prec == 1) {
     bool fallback = false, restart = false;
+    const int iT0 = c->iT_cur;  // the FP64 copy of the t the FP32 phase starts from
     int rc = solve_mixed(c, maxstep, tol, istep, rms, conver, low_steps, fallback, restart);
     if (rc) return rc;
     last = istep;
     diverge = !std::isfinite(rms) && !restart;
-    if (restart) {  // FP64 from the initial t (d.T[0]) with a fresh accelerator
+    if (restart) {
+      // FP64 from the t the FP32 phase started from (the FP32 phase leaves
+      // the FP64 level untouched) with a fresh accelerator; the FP32 steps
+      // spent count against maxstep
       c->prec = 0;
       reset_an2(c);
+      if (iT0 != c->iT_cur)
+        HIPCHECK(hipMemcpyAsync(c->d.T[c->iT_cur], c->d.T[iT0], sizeof(double) * c->N * c->natv,
+                                hipMemcpyDeviceToDevice, c->st));
     } else if (fallback) {  // plain FP64 steps from the last FP64 evaluation on
       if ((rc = accel_step<double>(c))) return rc;
     }
@@ -1627,8 +1635,12 @@ int rism_set_field(rism_ctx *c, int which, int site, const double *host) {
   const size_t off = (size_t)site * c->N;
   if (which == RISM_FIELD_T) {
     if (c->prec == 1) {
-      HIPCHECK(hipMemcpyAsync(c->scratch, host, sizeof(double) * c->N, hipMemcpyHostToDevice, c->st));
-      HIPCHECK(rism::run_convert((const double *)c->scratch, c->f.T[c->iT_cur] + off, c->N, c->st));
+      // the FP64 copy is kept too: a mixed solve whose FP32 phase overflows
+      // restarts in FP64 from it
+      HIPCHECK(hipMemcpyAsync(c->d.T[c->iT_cur] + off, host, sizeof(double) * c->N,
+                              hipMemcpyHostToDevice, c->st));
+      HIPCHECK(rism::run_convert((const double *)c->d.T[c->iT_cur] + off,
+                                 c->f.T[c->iT_cur] + off, c->N, c->st));
     } else {
       HIPCHECK(hipMemcpyAsync(c->d.T[c->iT_cur] + off, host, sizeof(double) * c->N,
                               hipMemcpyHostToDevice, c->st));

@@ -246,7 +246,7 @@ def per_kernel_table(ks, ab):
     return {k: {"launches": v[0], "avg_ms": v[1] / v[0],
                 "GB/s": (ab[k] / (v[1] / v[0] / 1000.0) / 1e9) if k in ab else None,
                 "algorithmic_GB": ab[k] / 1e9 if k in ab else None}
-            for k, v in ks.items()}
+            for k, v in ks.items() if v[0]}
 
 
 def phase_headline(args, budget, R):

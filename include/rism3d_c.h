@@ -211,7 +211,9 @@ int rism_thermo(rism_ctx *ctx, rism_thermo_out *out);
  * of nx*ny*nz doubles.  Replaces the D2H copies of iterate.cu:86-91. */
 int rism_get_field(rism_ctx *ctx, int which, int site, double *host);
 /* Uploads t (RISM_FIELD_T) or u (RISM_FIELD_U) of one site; used for
- * restart (read_tuv.cu, with the missing H2D copy done) and tests. */
+ * restart (read_tuv.cu, with the missing H2D copy done) and tests.  A mixed
+ * context at its FP32 level keeps the FP64 value of t as well: a solve whose
+ * FP32 phase overflows restarts in FP64 from it. */
 int rism_set_field(rism_ctx *ctx, int which, int site, const double *host);
 /* Timing of the last rism_solve / rism_run. */
 int rism_get_timing(rism_ctx *ctx, rism_timing *out);

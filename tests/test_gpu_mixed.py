@@ -133,3 +133,35 @@ def test_mixed_fp32_overflow_restarts_in_fp64(rism, oracle, gpu, case24):
     assert mixed["steps_fp32"] >= 1
     assert mixed["status"] == fp64["status"]
     assert mixed["steps"] == mixed["steps_fp32"] + fp64["steps"]
+
+
+def test_mixed_restart_starts_from_the_t_set(rism, gpu, case24, capfd):
+    """A t set through rism_set_field in a mixed context (here half the
+    initial t) is also where the FP64 restart after an FP32 overflow starts:
+    the first FP64 residual after the overflow equals the first residual of
+    an FP64 context given the same t (with the set-up's t instead it would be
+    ~300 against ~1e10)."""
+    sol, s, d, su, grid, box = case24
+    su = dict(su, uq=su["uq"] * 0.3)
+    runs = {}
+    x = None
+    for prec in (0, 1):
+        g = rism.RISM3D(grid, box, d, su, closure=rism.HNC, precision=prec, verbose=True)
+        g.initialize()
+        if x is None:
+            x = 0.5 * g.fields(rism.FIELD_T).reshape(g.natv, -1)
+        for v in range(g.natv):
+            g.set(rism.FIELD_T, v, x[v])
+        capfd.readouterr()
+        g.iterate(maxstep=8, tol=1e-6)
+        out = capfd.readouterr().out
+        g.close()
+        runs[prec] = [ln.split() for ln in out.splitlines() if ln.strip().startswith("Step =")]
+        print(prec, runs[prec])
+    first64 = runs[0][0][5]
+    mixed = runs[1]
+    k = next(i for i, w in enumerate(mixed) if "overflow:" in w)
+    assert k + 1 < len(mixed)
+    after = mixed[k + 1]
+    assert after[3] == "Reside" and "(FP32)" not in after
+    assert after[5] == first64, (after, first64)
