@@ -14,6 +14,7 @@
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -193,6 +194,10 @@ struct rism_ctx {
   // and hflag are their pinned host mirrors
   double *scal = nullptr, *rms_hist = nullptr, *rho_dev = nullptr, *hscal = nullptr;
   int *dflag = nullptr, *hflag = nullptr;
+  // FNV-1a hash of the solvent's data (sites, chi table, T, compressibility):
+  // contexts with equal grids, boxes, options and solvents share every table
+  // and step together in batches
+  uint64_t solv_hash = 0;
   size_t rms_cap = 0;
   bool async = false;
   // one period of mixing steps captured as a graph (see async_enqueue), with
@@ -1004,6 +1009,292 @@ int async_check(AsyncRun &a) {
   return RISM_OK;
 }
 
+// ---- grouped batch solves -------------------------------------------------
+// FP64 AN2 contexts of one grid shape, box, solvent and options are stepped
+// together: every kernel of a step is launched once for the group, one
+// context per blockIdx.z, with each context's pointers taken from a device
+// table (rism::GroupPtrs).  All members start from the same accelerator
+// state and follow the same slot rotation, so one table per state serves a
+// step of every member, and a 6-step period of grouped launches is captured
+// once per group as a hipGraph.  A member that converges or diverges raises
+// its own flag and its workgroups return at entry; the group runs until
+// every member has stopped.  Per context the arithmetic is that of
+// calc_step + an2_step, so the results equal separate solves bit for bit.
+constexpr int kMaxGroup = 16;
+
+// the accelerator's slot indices of a group (as in rism_ctx)
+struct GroupIdx {
+  int iT, iR, s0, s1, b, count;
+  bool operator<(const GroupIdx &o) const {
+    return std::tie(iT, iR, s0, s1, b, count) < std::tie(o.iT, o.iR, o.s0, o.s1, o.b, o.count);
+  }
+  bool operator==(const GroupIdx &o) const { return !(*this < o) && !(o < *this); }
+};
+
+GroupIdx group_idx(const rism_ctx *c) {
+  return GroupIdx{c->iT_cur, c->iR_cur, c->iR_slot[0], c->iR_slot[1], c->binary, c->count};
+}
+
+// contexts whose steps can share launches: equal kernel parameters (grid,
+// box, solvent, closure, chi source, launch plans), AN2 options and state
+bool groupable(const rism_ctx *a, const rism_ctx *b) {
+  const rism::Params &p = a->P, &q = b->P;
+  return a->nx == b->nx && a->ny == b->ny && a->nz == b->nz && a->natv == b->natv &&
+         a->box[0] == b->box[0] && a->box[1] == b->box[1] && a->box[2] == b->box[2] &&
+         a->solv_hash == b->solv_hash && p.clos == q.clos && p.shell == q.shell &&
+         a->opts.an2_count == b->opts.an2_count && a->opts.an2_m == b->opts.an2_m &&
+         a->opts.an2_mp == b->opts.an2_mp && p.var_zf[0] == q.var_zf[0] &&
+         p.var_zi[0] == q.var_zi[0] && p.var_y[0] == q.var_y[0] && a->prec == 0 &&
+         b->prec == 0 && group_idx(a) == group_idx(b);
+}
+
+struct GroupRun {
+  std::vector<rism_ctx *> m;
+  std::vector<AsyncRun *> runs;
+  GroupIdx ix{};
+  int istep = 0, maxstep = 0;
+  double tol = 0.0;
+  bool finished = false;
+  // device tables by state, in one allocation of kMaxStates tables; their
+  // host images stay alive until the group ends (asynchronous uploads)
+  static constexpr int kMaxStates = 32;
+  rism::GroupPtrs *dtab = nullptr;
+  std::map<GroupIdx, int> tabs;
+  std::vector<std::vector<rism::GroupPtrs>> host;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  GroupIdx gix{};
+  GroupRun() = default;
+  GroupRun(const GroupRun &) = delete;
+  GroupRun &operator=(const GroupRun &) = delete;
+  ~GroupRun() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (dtab) (void)hipFree(dtab);
+  }
+  rism_ctx *c0() const { return m[0]; }
+};
+
+// the device table of state ix: each member's buffers for one step
+int group_table(GroupRun &g, const GroupIdx &ix, rism::GroupPtrs **out) {
+  const size_t k = g.m.size();
+  auto it = g.tabs.find(ix);
+  if (it != g.tabs.end()) {
+    *out = g.dtab + (size_t)it->second * k;
+    return RISM_OK;
+  }
+  const int slot = (int)g.host.size();
+  if (slot >= GroupRun::kMaxStates)
+    return rism::set_error(RISM_ERR_STATE, "too many accelerator states in a group");
+  g.host.emplace_back(k, rism::GroupPtrs{});
+  std::vector<rism::GroupPtrs> &h = g.host.back();
+  for (size_t i = 0; i < k; ++i) {
+    rism_ctx *c = g.m[i];
+    Level<double> &L = c->d;
+    rism::GroupPtrs &e = h[i];
+    e.t = L.T[ix.iT];
+    e.u = L.u;
+    e.fr = L.fr;
+    e.spec = L.spec;
+    e.fk = L.fk;
+    e.R = L.R[ix.iR];
+    e.R0 = L.R[ix.s0];
+    e.R1 = L.R[ix.s1];
+    e.partial = c->partial;
+    e.sums = c->sums;
+    e.scal = c->scal;
+    e.rms_hist = c->rms_hist;
+    e.done = c->dflag;
+    e.tnew = L.T[1 - ix.iT];
+    e.P0 = L.Pslot[0];
+    e.P1 = L.Pslot[1];
+    e.Pb = L.Pslot[ix.b];
+  }
+  rism::GroupPtrs *d = g.dtab + (size_t)slot * k;
+  g.tabs[ix] = slot;
+  // ordered on the group's stream before the launches that read it (never
+  // called under capture: group_graph makes the period's tables first)
+  HIPCHECK(hipMemcpyAsync(d, h.data(), sizeof(rism::GroupPtrs) * k, hipMemcpyHostToDevice,
+                          g.c0()->st));
+  *out = d;
+  return RISM_OK;
+}
+
+// one step (calc_step) and one AN2 update (an2_step) of every member at the
+// group's state g.ix, which then advances as an2_step advances a context's
+int group_step(GroupRun &g) {
+  rism_ctx *c = g.c0();
+  rism::GroupPtrs *tab = nullptr;
+  int rc = group_table(g, g.ix, &tab);
+  if (rc) return rc;
+  Level<double> &L = c->d;
+  const GroupIdx ix = g.ix;
+  rism::Params P = c->P;
+  P.mode = 0;
+  P.done = nullptr;
+  P.grp = tab;
+  P.ngrp = (int)g.m.size();
+  hipStream_t st = c->st;
+  // the pointer arguments are the first member's (its tables serve all)
+  HIPCHECK(rism::run_zfwd(P, L.T[ix.iT], L.u, L.fr, L.spec, L.tb.twz, L.tb.zpre, L.tb.zpost, st));
+  HIPCHECK(rism::run_y(P, L.spec, -1, L.tb.twy, L.tb.prey, st));
+  HIPCHECK(rism::run_xoz(P, L.spec, L.fk, L.tb.twx, L.tb.prex, c->knots, c->coef, L.chitab, st));
+  HIPCHECK(rism::run_y(P, L.spec, +1, L.tb.twy, L.tb.prey, st));
+  HIPCHECK(rism::run_zinv(P, L.spec, L.T[ix.iT], L.u, L.R[ix.iR], L.R[ix.s0], L.R[ix.s1],
+                          (const double *)nullptr, c->partial, L.tb.twz, L.tb.zpre, L.tb.zpost,
+                          ix.count == 0, st));
+  const size_t nb = rism::zinv_num_blocks<double>(P, 0) / c->natv;
+  HIPCHECK(rism::run_reduce_cols(c->partial, nb, 6, c->natv, c->sums, nullptr, st, tab, P.ngrp));
+  HIPCHECK(rism::run_step_finish(c->sums, c->rho_dev, c->natv, c->N, g.tol, ix.count == 0,
+                                 c->opts.an2_mp, ix.b, c->scal, c->rms_hist, c->dflag, st, tab,
+                                 P.ngrp));
+  const size_t nn = c->N * c->natv;
+  if (ix.count > 0) {
+    HIPCHECK(rism::run_an2_picard(L.T[1 - ix.iT], L.T[ix.iT], L.R[ix.iR], L.Pslot[ix.b],
+                                  c->opts.an2_m, nn, (const int *)nullptr, st, tab, P.ngrp));
+  } else {
+    HIPCHECK(rism::run_an2_update(L.T[1 - ix.iT], L.T[ix.iT], L.R[ix.iR], L.Pslot[0],
+                                  L.Pslot[1], L.Pslot[ix.b], c->scal, c->opts.an2_m, nn,
+                                  (const int *)nullptr, st, tab, P.ngrp));
+  }
+  // an2_step's bookkeeping
+  GroupIdx &n = g.ix;
+  if (n.count > 0) --n.count;
+  int *slot = n.b == 0 ? &n.s0 : &n.s1;
+  const int oldR = *slot;
+  *slot = n.iR;
+  n.iR = oldR;
+  n.iT = 1 - n.iT;
+  n.b = n.b == 0 ? 1 : 0;
+  return RISM_OK;
+}
+
+// the 6-step period from the current state as one graph (capture runs
+// nothing; the state is restored afterwards)
+int group_graph(GroupRun &g) {
+  if (g.gexec && g.gix == g.ix) return RISM_OK;
+  if (g.gexec) (void)hipGraphExecDestroy(g.gexec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.gexec = nullptr;
+  g.graph = nullptr;
+  // the tables first: their copies must not be captured
+  GroupIdx probe = g.ix;
+  {
+    const GroupIdx keep = g.ix;
+    for (int k = 0; k < kGraphSteps; ++k) {
+      rism::GroupPtrs *t = nullptr;
+      int rc = group_table(g, g.ix, &t);
+      if (rc) return rc;
+      GroupIdx &n = g.ix;  // advance as group_step does
+      int *slot = n.b == 0 ? &n.s0 : &n.s1;
+      const int oldR = *slot;
+      *slot = n.iR;
+      n.iR = oldR;
+      n.iT = 1 - n.iT;
+      n.b = n.b == 0 ? 1 : 0;
+    }
+    g.ix = keep;
+  }
+  hipStream_t st = g.c0()->st;
+  HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = RISM_OK;
+  for (int k = 0; k < kGraphSteps && !rc; ++k) rc = group_step(g);
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &gr);
+  g.ix = probe;
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return rism::set_error(RISM_ERR_HIP, "hipStreamEndCapture failed");
+  g.graph = gr;
+  HIPCHECK(hipGraphInstantiate(&g.gexec, g.graph, nullptr, nullptr, 0));
+  g.gix = g.ix;
+  return RISM_OK;
+}
+
+int group_begin(GroupRun &g, int maxstep, double tol) {
+  g.maxstep = maxstep;
+  g.tol = tol;
+  g.istep = 0;
+  g.ix = group_idx(g.c0());
+  if (hipMalloc(&g.dtab, sizeof(rism::GroupPtrs) * GroupRun::kMaxStates * g.m.size()) !=
+      hipSuccess)
+    return rism::set_error(RISM_ERR_NOMEM, "hipMalloc of the group tables failed");
+  for (size_t i = 0; i < g.m.size(); ++i) {
+    int rc = async_begin(*g.runs[i], g.m[i], maxstep, tol);
+    if (rc) return rc;
+  }
+  // the members' set-up and flag resets ran on their own streams
+  for (rism_ctx *c : g.m)
+    if (hipStreamSynchronize(c->st) != hipSuccess)
+      return rism::set_error(RISM_ERR_HIP, "hipStreamSynchronize failed");
+  return RISM_OK;
+}
+
+// enqueues up to `chunk` steps of the group and the read-back of every
+// running member's flag (async_enqueue for a group)
+int group_enqueue(GroupRun &g, int chunk) {
+  rism_ctx *c = g.c0();
+  auto log_step = [&]() {
+    ++g.istep;
+    for (size_t i = 0; i < g.m.size(); ++i)
+      if (!g.runs[i]->finished)
+        g.runs[i]->log[g.istep] =
+            AsyncRun::Log{g.ix.iT, g.ix.iR, g.ix.s0, g.ix.s1, g.ix.b, g.ix.count, 0, 0, {}};
+  };
+  int rc;
+  for (int k = 0; k < chunk && g.istep < g.maxstep;) {
+    if (g.ix.count == 0 && k + kGraphSteps <= chunk && g.istep + kGraphSteps <= g.maxstep) {
+      if ((rc = group_graph(g))) return rc;
+      HIPCHECK(hipGraphLaunch(g.gexec, c->st));
+      for (int j = 0; j < kGraphSteps; ++j) {
+        log_step();
+        GroupIdx &n = g.ix;
+        int *slot = n.b == 0 ? &n.s0 : &n.s1;
+        const int oldR = *slot;
+        *slot = n.iR;
+        n.iR = oldR;
+        n.iT = 1 - n.iT;
+        n.b = n.b == 0 ? 1 : 0;
+      }
+      k += kGraphSteps;
+      continue;
+    }
+    log_step();
+    if ((rc = group_step(g))) return rc;
+    ++k;
+  }
+  for (size_t i = 0; i < g.m.size(); ++i)
+    if (!g.runs[i]->finished)
+      HIPCHECK(hipMemcpyAsync(g.m[i]->hflag, g.m[i]->dflag, 2 * sizeof(int),
+                              hipMemcpyDeviceToHost, c->st));
+  return RISM_OK;
+}
+
+// after the group's stream has drained: each running member takes the
+// group's state and step count, then async_check decides it
+int group_check(GroupRun &g) {
+  bool left = false;
+  for (size_t i = 0; i < g.m.size(); ++i) {
+    AsyncRun &a = *g.runs[i];
+    if (a.finished) continue;
+    rism_ctx *c = g.m[i];
+    c->iT_cur = g.ix.iT;
+    c->iR_cur = g.ix.iR;
+    c->iR_slot[0] = g.ix.s0;
+    c->iR_slot[1] = g.ix.s1;
+    c->binary = g.ix.b;
+    c->count = g.ix.count;
+    a.istep = g.istep;
+    int rc = async_check<double>(a);
+    if (rc) return rc;
+    left = left || !a.finished;
+  }
+  g.finished = !left;
+  return RISM_OK;
+}
+
 int setup_enqueue(rism_ctx *c) {
   const size_t nn = c->N * c->natv;
   c->prec = 0;
@@ -1321,6 +1612,23 @@ rism_ctx *rism_create(const rism_grid *grid, const rism_solvent *solvent,
   c->qv.assign(solvent->q, solvent->q + c->natv);
   c->sigv.assign(solvent->sig, solvent->sig + c->natv);
   c->epsv.assign(solvent->eps, solvent->eps + c->natv);
+  {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t bytes) {
+      const unsigned char *q = static_cast<const unsigned char *>(p);
+      for (size_t i = 0; i < bytes; ++i) h = (h ^ q[i]) * 1099511628211ull;
+    };
+    const size_t nv = (size_t)solvent->natv;
+    mix(&solvent->natv, sizeof(int));
+    mix(&solvent->ntab, sizeof(int));
+    mix(&solvent->dr, sizeof(double));
+    mix(&solvent->temper, sizeof(double));
+    mix(&solvent->xt, sizeof(double));
+    for (const double *a : {solvent->sig, solvent->eps, solvent->q, solvent->rho})
+      mix(a, sizeof(double) * nv);
+    mix(solvent->xvv, sizeof(double) * nv * nv * (size_t)solvent->ntab);
+    c->solv_hash = h;
+  }
   c->natu = solute->natu;
   c->usig.assign(solute->sig, solute->sig + c->natu);
   c->ueps.assign(solute->eps, solute->eps + c->natu);
@@ -1675,16 +1983,52 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
   const int nt = std::max(1, std::min(threads > 0 ? threads : 1, n));
   std::vector<int> rcs(nt, RISM_OK);
   std::vector<std::string> msgs(nt);  // error messages are per thread
-  // worker w drives contexts w, w + nt, ...  FP64 AN2 contexts run without a
-  // host round trip per step: enqueue a chunk on each of their streams, then
-  // drain them in turn, until every run has finished.  Contexts whose
-  // accelerator solves on the host (mixed precision, Anderson(m)) then run
-  // rism_solve's step-by-step loop on their own streams, concurrently with
-  // the other workers' contexts.
+  // FP64 AN2 contexts run without a host round trip per step, in groups of
+  // equal grid, box, solvent and options (one launch per kernel and group,
+  // up to kMaxGroup contexts); other FP64 contexts that can run sync-free
+  // (Anderson(m), kernel timing on) run so one by one; contexts whose steps
+  // need the host (mixed precision, verbose) run rism_solve's loop.  The
+  // groups and contexts are dealt to the worker threads, each of which
+  // enqueues a chunk on every one of its groups and contexts, then drains
+  // them in turn, until every run has finished.
+  std::vector<std::vector<int>> groups;
+  std::vector<int> single, slow;
+  for (int i = 0; i < n; ++i) {
+    rism_ctx *c = ctx[i];
+    if (!async_capable(c)) {
+      slow.push_back(i);
+    } else if (c->opts.anderson_m > 0 || c->ktime || c->prec != 0) {
+      single.push_back(i);
+    } else {
+      bool put = false;
+      for (auto &gr : groups)
+        if ((int)gr.size() < kMaxGroup && groupable(ctx[gr[0]], c)) {
+          gr.push_back(i);
+          put = true;
+          break;
+        }
+      if (!put) groups.push_back({i});
+    }
+  }
+  std::vector<std::unique_ptr<GroupRun>> gruns(groups.size());
   auto work = [&](int w) {
     int rc = RISM_OK;
-    std::vector<int> fast, slow;
-    for (int i = w; i < n; i += nt) (async_capable(ctx[i]) ? fast : slow).push_back(i);
+    std::vector<GroupRun *> mine;
+    for (size_t q = w; q < groups.size(); q += nt) {
+      gruns[q].reset(new GroupRun());
+      for (int i : groups[q]) {
+        gruns[q]->m.push_back(ctx[i]);
+        gruns[q]->runs.push_back(&runs[i]);
+      }
+      mine.push_back(gruns[q].get());
+    }
+    std::vector<int> fast, slw;
+    for (size_t q = w; q < single.size(); q += nt) fast.push_back(single[q]);
+    for (size_t q = w; q < slow.size(); q += nt) slw.push_back(slow[q]);
+    for (GroupRun *g : mine) {
+      if (rc) break;
+      rc = group_begin(*g, maxstep, tol);
+    }
     for (int i : fast) {
       if (rc) break;
       rc = async_begin(runs[i], ctx[i], maxstep, tol);
@@ -1692,9 +2036,21 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
     bool left = true;
     while (!rc && left) {
       left = false;
+      for (GroupRun *g : mine) {
+        if (rc) break;
+        if (!g->finished) rc = group_enqueue(*g, kAsyncChunk);
+      }
       for (int i : fast) {
         if (rc) break;
         if (!runs[i].finished) rc = async_enqueue<double>(runs[i], kAsyncChunk);
+      }
+      for (GroupRun *g : mine) {
+        if (rc) break;
+        if (g->finished) continue;
+        if (hipStreamSynchronize(g->c0()->st) != hipSuccess)
+          rc = rism::set_error(RISM_ERR_HIP, "hipStreamSynchronize failed");
+        if (!rc) rc = group_check(*g);
+        left = left || !g->finished;
       }
       for (int i : fast) {
         if (rc) break;
@@ -1705,8 +2061,13 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
         left = left || !runs[i].finished;
       }
     }
+    for (GroupRun *g : mine) {
+      // a failed group must not leave work in flight on its stream
+      (void)hipStreamSynchronize(g->c0()->st);
+      for (rism_ctx *c : g->m) c->async = false;
+    }
     for (int i : fast) ctx[i]->async = false;
-    for (int i : slow) {
+    for (int i : slw) {
       if (rc) break;
       rc = rism_solve(ctx[i], maxstep, tol, &runs[i].status, &runs[i].steps, &runs[i].rms);
     }

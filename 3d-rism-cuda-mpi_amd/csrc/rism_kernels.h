@@ -15,6 +15,24 @@ using cplxf = cx<float>;
 
 // Everything a kernel needs to know about the grid, the solvent sites and
 // the chi spline, passed by value.
+// Grouped launches: contexts of one grid shape, solvent and options stepped
+// together by one launch per kernel, one context per blockIdx.z.  Each
+// kernel then takes the pointers of its context from grp[blockIdx.z]; its
+// own pointer arguments are the first context's (the twiddle and chi tables
+// are equal across the group).
+struct GroupPtrs {
+  const void *t, *u, *fr;  // forward z pass (bulk type)
+  void *spec;
+  const void *fk;  // x pass
+  void *R;         // residual pass: output and the two AN2 slot residuals
+  const void *R0, *R1;
+  double *partial, *sums, *scal, *rms_hist;
+  int *done;
+  void *tnew;  // AN2 update
+  const void *P0, *P1;
+  void *Pb;
+};
+
 struct Params {
   int nx, ny, nz;
   int natv;
@@ -36,6 +54,9 @@ struct Params {
   // launch plans of the forward z, residual z and y passes picked by the
   // autotuner, per bulk type ([0] FP64, [1] FP32); 0 is the default plan
   int var_zf[2], var_zi[2], var_y[2];
+  // grouped launch (GroupPtrs): ngrp contexts, or grp = null and ngrp = 0
+  const GroupPtrs *grp;
+  int ngrp;
 };
 
 // Launch plans the autotuner chooses between (rism_ctx.cpp): whether plan
@@ -116,7 +137,8 @@ hipError_t run_andm_finish(const double *sums, const double *rho, int natv, size
                            hipStream_t st);
 // nsets consecutive blocks of nrow x width partial sums -> nsets x width
 hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, double *out,
-                           const int *done, hipStream_t st);
+                           const int *done, hipStream_t st, const GroupPtrs *grp = nullptr,
+                           int ngrp = 0);
 // End of one AN2 step on the device (cal_rms.cu:20-24 and, when the next
 // update mixes, cal_theta + leg of anderson.cu:81-130): from the per-site
 // sums (6 per site) it writes scal = {rms, x0, x1, s1, s2}.  With a flag
@@ -127,15 +149,18 @@ hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, 
 // next update retires.
 hipError_t run_step_finish(const double *sums, const double *rho, int natv, size_t N,
                            double tol, int mixing, double mp, int binary, double *scal,
-                           double *rms_hist, int *done, hipStream_t st);
+                           double *rms_hist, int *done, hipStream_t st,
+                           const GroupPtrs *grp = nullptr, int ngrp = 0);
 template <class Re>
 hipError_t run_an2_picard(Re *tnew, const Re *t, const Re *R, Re *Pb, double m, size_t n,
-                          const int *done, hipStream_t st);
+                          const int *done, hipStream_t st, const GroupPtrs *grp = nullptr,
+                          int ngrp = 0);
 // s1, s2 from scal[3], scal[4] (run_step_finish)
 template <class Re>
 hipError_t run_an2_update(Re *tnew, const Re *t, const Re *R, const Re *P0, const Re *P1, Re *Pb,
                           const double *scal, double m, size_t n, const int *done,
-                          hipStream_t st);
+                          hipStream_t st,
+                          const GroupPtrs *grp = nullptr, int ngrp = 0);
 template <class From, class To>
 hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st);
 // closure exponent a = -u + t at the base point t0 of a correction sweep,

@@ -14,8 +14,14 @@ namespace rism {
 // Kahan-compensated per thread, pairwise tree within the workgroup).
 __global__ void __launch_bounds__(256)
     k_reduce_cols(const double *__restrict__ in, size_t nrow, int width,
-                  double *__restrict__ out, const int *done) {
+                  double *__restrict__ out, const int *done, const GroupPtrs *grp) {
   __shared__ double sh[256];
+  if (grp) {  // grouped launch: this context's partial sums
+    const GroupPtrs &g = grp[blockIdx.z];
+    done = g.done;
+    in = g.partial;
+    out = g.sums;
+  }
   if (done && *done) return;
   const int col = blockIdx.x;
   in += (size_t)blockIdx.y * nrow * width;
@@ -92,7 +98,15 @@ __device__ __forceinline__ void st_nt(V16<Re> *p, const V16<Re> &x) {
 template <class Re>
 __global__ void k_an2_picard(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
                              const V16<Re> *__restrict__ R, V16<Re> *__restrict__ Pb, Re mcoef,
-                             size_t nv, const int *done) {
+                             size_t nv, const int *done, const GroupPtrs *grp) {
+  if (grp) {
+    const GroupPtrs &g = grp[blockIdx.z];
+    done = g.done;
+    tnew = (V16<Re> *)g.tnew;
+    t = (const V16<Re> *)g.t;
+    R = (const V16<Re> *)g.R;
+    Pb = (V16<Re> *)g.Pb;
+  }
   if (done && *done) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
@@ -113,7 +127,18 @@ template <class Re>
 __global__ void k_an2_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restrict__ t,
                              const V16<Re> *__restrict__ R, const V16<Re> *P0,
                              const V16<Re> *P1, V16<Re> *Pb, const double *__restrict__ scal,
-                             Re mcoef, size_t nv, const int *done) {
+                             Re mcoef, size_t nv, const int *done, const GroupPtrs *grp) {
+  if (grp) {
+    const GroupPtrs &g = grp[blockIdx.z];
+    done = g.done;
+    tnew = (V16<Re> *)g.tnew;
+    t = (const V16<Re> *)g.t;
+    R = (const V16<Re> *)g.R;
+    P0 = (const V16<Re> *)g.P0;
+    P1 = (const V16<Re> *)g.P1;
+    Pb = (V16<Re> *)g.Pb;
+    scal = g.scal;
+  }
   if (done && *done) return;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
@@ -137,8 +162,15 @@ __global__ void k_an2_update(V16<Re> *__restrict__ tnew, const V16<Re> *__restri
 __global__ void k_step_finish(const double *__restrict__ sums, const double *__restrict__ rho,
                               int natv, size_t N, double tol, int mixing, double mp, int binary,
                               double *__restrict__ scal, double *__restrict__ rms_hist,
-                              int *__restrict__ done) {
+                              int *__restrict__ done, const GroupPtrs *grp) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (grp) {
+    const GroupPtrs &g = grp[blockIdx.z];
+    sums = g.sums;
+    scal = g.scal;
+    rms_hist = g.rms_hist;
+    done = g.done;
+  }
   if (done && done[0]) return;
   // step number: done[2] counts the finished steps of the run
   int step = 0;
@@ -789,17 +821,18 @@ hipError_t run_copy(const double *in, double *out, size_t n, hipStream_t st) {
 }
 
 hipError_t run_reduce_cols(const double *in, size_t nrow, int width, int nsets, double *out,
-                           const int *done, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_cols, dim3(width, nsets), dim3(256), 0, st, in, nrow, width, out,
-                     done);
+                           const int *done, hipStream_t st, const GroupPtrs *grp, int ngrp) {
+  hipLaunchKernelGGL(k_reduce_cols, dim3(width, nsets, grp ? ngrp : 1), dim3(256), 0, st, in,
+                     nrow, width, out, done, grp);
   return hipGetLastError();
 }
 
 hipError_t run_step_finish(const double *sums, const double *rho, int natv, size_t N,
                            double tol, int mixing, double mp, int binary, double *scal,
-                           double *rms_hist, int *done, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_finish, dim3(1), dim3(64), 0, st, sums, rho, natv, N, tol, mixing,
-                     mp, binary, scal, rms_hist, done);
+                           double *rms_hist, int *done, hipStream_t st, const GroupPtrs *grp,
+                           int ngrp) {
+  hipLaunchKernelGGL(k_step_finish, dim3(1, 1, grp ? ngrp : 1), dim3(64), 0, st, sums, rho, natv,
+                     N, tol, mixing, mp, binary, scal, rms_hist, done, grp);
   return hipGetLastError();
 }
 
@@ -810,23 +843,25 @@ static size_t nvec(size_t n) {
 }
 template <class Re>
 hipError_t run_an2_picard(Re *tnew, const Re *t, const Re *R, Re *Pb, double m, size_t n,
-                          const int *done, hipStream_t st) {
+                          const int *done, hipStream_t st, const GroupPtrs *grp, int ngrp) {
   if (n % V16<Re>::K) return hipErrorInvalidValue;
   const size_t nv = nvec<Re>(n);
-  hipLaunchKernelGGL(k_an2_picard<Re>, dim3(nblk(nv, 256)), dim3(256), 0, st, (V16<Re> *)tnew,
-                     (const V16<Re> *)t, (const V16<Re> *)R, (V16<Re> *)Pb, (Re)m, nv, done);
+  hipLaunchKernelGGL(k_an2_picard<Re>, dim3(nblk(nv, 256), 1, grp ? ngrp : 1), dim3(256), 0, st,
+                     (V16<Re> *)tnew, (const V16<Re> *)t, (const V16<Re> *)R, (V16<Re> *)Pb,
+                     (Re)m, nv, done, grp);
   return hipGetLastError();
 }
 
 template <class Re>
 hipError_t run_an2_update(Re *tnew, const Re *t, const Re *R, const Re *P0, const Re *P1, Re *Pb,
                           const double *scal, double m, size_t n, const int *done,
-                          hipStream_t st) {
+                          hipStream_t st, const GroupPtrs *grp, int ngrp) {
   if (n % V16<Re>::K) return hipErrorInvalidValue;
   const size_t nv = nvec<Re>(n);
-  hipLaunchKernelGGL(k_an2_update<Re>, dim3(nblk(nv, 256)), dim3(256), 0, st, (V16<Re> *)tnew,
-                     (const V16<Re> *)t, (const V16<Re> *)R, (const V16<Re> *)P0,
-                     (const V16<Re> *)P1, (V16<Re> *)Pb, scal, (Re)m, nv, done);
+  hipLaunchKernelGGL(k_an2_update<Re>, dim3(nblk(nv, 256), 1, grp ? ngrp : 1), dim3(256), 0, st,
+                     (V16<Re> *)tnew, (const V16<Re> *)t, (const V16<Re> *)R,
+                     (const V16<Re> *)P0, (const V16<Re> *)P1, (V16<Re> *)Pb, scal, (Re)m, nv,
+                     done, grp);
   return hipGetLastError();
 }
 
@@ -864,10 +899,10 @@ hipError_t run_convert(const From *in, To *out, size_t n, hipStream_t st) {
 
 #define INST(Re)                                                                              \
   template hipError_t run_an2_picard<Re>(Re *, const Re *, const Re *, Re *, double, size_t,  \
-                                         const int *, hipStream_t);                           \
+                                         const int *, hipStream_t, const GroupPtrs *, int);   \
   template hipError_t run_an2_update<Re>(Re *, const Re *, const Re *, const Re *, const Re *, \
                                          Re *, const double *, double, size_t, const int *,   \
-                                         hipStream_t);                                        \
+                                         hipStream_t, const GroupPtrs *, int);                \
   template hipError_t run_andm_update<Re>(Re *, const Re *, const Re *,                       \
                                           const HistUpdateT<Re> &, Re *, const double *,      \
                                           double, size_t, const int *, hipStream_t);          \

@@ -169,7 +169,12 @@ __global__ void __launch_bounds__(512)
   using C = cx<Re>;
   constexpr int T = FftShape<L>::T, CS = FftShape<L>::CS;
   C *lds = reinterpret_cast<C *>(xsm);
-  if (P.done && *P.done) return;
+  if (P.grp) {  // grouped launch: this context's spectrum and fk
+    const GroupPtrs &g = P.grp[blockIdx.z];
+    if (g.done && *g.done) return;
+    spec = (C *)g.spec;
+    fkraw = (const C *)g.fk;
+  } else if (P.done && *P.done) return;
   const int tid = threadIdx.x;
   const int nv = P.natv;
   const int v = tid / T, j = tid % T;
@@ -291,7 +296,7 @@ template <int L, class Re>
 static void launch_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const cx<Re> *twL,
                        const cx<Re> *prex, const double *knots, const double *coef,
                        const Re *chitab, hipStream_t st) {
-  dim3 grid(P.ny, P.nz / 2);
+  dim3 grid(P.ny, P.nz / 2, P.grp ? P.ngrp : 1);
   const int threads = P.natv * FftShape<L>::T;
   const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cx<Re>);
   if constexpr (sizeof(Re) == sizeof(double)) {

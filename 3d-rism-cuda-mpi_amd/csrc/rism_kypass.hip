@@ -11,10 +11,15 @@ namespace rism {
 template <int L, int B, int S, class Re>
 __global__ void __launch_bounds__(B *FftShape<L>::T)
     k_ypass(cx<Re> *__restrict__ spec, int nx, const cx<Re> *__restrict__ twL,
-            const cx<Re> *__restrict__ pre, const int *done) {
+            const cx<Re> *__restrict__ pre, const int *done, const GroupPtrs *grp) {
   using C = cx<Re>;
   constexpr int CS = FftShape<L>::CS;
   __shared__ C lds[B * CS];
+  if (grp) {  // grouped launch: this context's spectrum
+    const GroupPtrs &g = grp[blockIdx.z];
+    done = g.done;
+    spec = (C *)g.spec;
+  }
   if (done && *done) return;
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
@@ -67,9 +72,9 @@ template <int L, int S, class Re, int V = 0>
 static void launch_y(const Params &P, cx<Re> *spec, int nplanes, const cx<Re> *twL,
                      const cx<Re> *pre, hipStream_t st) {
   constexpr int B = YPlan<L, Re, V>::B;
-  dim3 grid((P.nx + B - 1) / B, nplanes);
+  dim3 grid((P.nx + B - 1) / B, nplanes, P.grp ? P.ngrp : 1);
   hipLaunchKernelGGL((k_ypass<L, B, S, Re>), grid, dim3(B * FftShape<L>::T), 0, st, spec, P.nx,
-                     twL, pre, P.done);
+                     twL, pre, P.done, P.grp);
 }
 template <int L, int S, class Re>
 static void launch_y_plan(const Params &P, cx<Re> *spec, int nplanes, const cx<Re> *twL,

@@ -39,7 +39,14 @@ __global__ void __launch_bounds__(B *TZ)
   using C = cx<Re>;
   constexpr int T = TZ, CS = FftShape<M>::CS;
   __shared__ C lds[B * CS];
-  if (P.done && *P.done) return;
+  if (P.grp) {  // grouped launch: this context's fields
+    const GroupPtrs &g = P.grp[blockIdx.z];
+    if (g.done && *g.done) return;
+    t = (const Re *)g.t;
+    u = (const Re *)g.u;
+    fr = (const Re *)g.fr;
+    spec = (C *)g.spec;
+  } else if (P.done && *P.done) return;
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   // logical workgroup order: site fastest, then x tile, then y (sites of
@@ -150,7 +157,17 @@ __global__ void __launch_bounds__(B *TZ)
   constexpr int NC = (int)((NRED * W * sizeof(double) + sizeof(C) - 1) / sizeof(C));
   constexpr int LDSN = (B * CS > NC) ? B * CS : NC;
   __shared__ C lds[LDSN];
-  if (P.done && *P.done) return;
+  if (P.grp) {  // grouped launch (AN2 and Picard steps): this context's fields
+    const GroupPtrs &g = P.grp[blockIdx.z];
+    if (g.done && *g.done) return;
+    spec = (const C *)g.spec;
+    t = (const Re *)g.t;
+    u = (const Re *)g.u;
+    R = (Re *)g.R;
+    R0 = (const Re *)g.R0;
+    R1 = (const Re *)g.R1;
+    partial = g.partial;
+  } else if (P.done && *P.done) return;
   const int tid = threadIdx.x;
   const int c = tid % B, j = tid / B;
   const int ntx = (P.nx + B - 1) / B;
@@ -418,8 +435,8 @@ static void launch_zfwd(const Params &P, const Re *t, const Re *u, const Re *fr,
                         hipStream_t st) {
   using Z = ZfPlan<M, Re, V>;
   hipLaunchKernelGGL((k_zfwd_closure<M, Z::B, Z::T, Z::MAXR, KIND, Re, Z::WPE>),
-                     dim3(zgrid(P, Z::B)), dim3(Z::B * Z::T), 0, st, t, u, fr, P, spec, twM,
-                     zpre, zpost);
+                     dim3(zgrid(P, Z::B), 1, P.grp ? P.ngrp : 1), dim3(Z::B * Z::T), 0, st, t, u,
+                     fr, P, spec, twM, zpre, zpost);
 }
 template <int M, int HM, int KIND, class Re, int V = 0>
 static void launch_zinv(const Params &P, const cx<Re> *spec, const Re *t, const Re *u, Re *R,
@@ -427,9 +444,9 @@ static void launch_zinv(const Params &P, const cx<Re> *spec, const Re *t, const 
                         double *partial, const cx<Re> *twM, const cx<Re> *zpre,
                         const cx<Re> *zpost, hipStream_t st) {
   using Z = ZiPlan<M, Re, HM, V>;
-  hipLaunchKernelGGL((k_zinv_resid<M, Z::B, Z::T, HM, KIND, Re, Z::WPE>), dim3(zgrid(P, Z::B)),
-                     dim3(Z::B * Z::T), 0, st, spec, t, u, R, R0, R1, r0, H, P, partial, twM,
-                     zpre, zpost);
+  hipLaunchKernelGGL((k_zinv_resid<M, Z::B, Z::T, HM, KIND, Re, Z::WPE>),
+                     dim3(zgrid(P, Z::B), 1, P.grp ? P.ngrp : 1), dim3(Z::B * Z::T), 0, st, spec,
+                     t, u, R, R0, R1, r0, H, P, partial, twM, zpre, zpost);
 }
 
 // Lengths with more than one plan: the autotuner's search space (smaller
