@@ -1035,6 +1035,18 @@ GroupIdx group_idx(const rism_ctx *c) {
   return GroupIdx{c->iT_cur, c->iR_cur, c->iR_slot[0], c->iR_slot[1], c->binary, c->count};
 }
 
+// the state after one AN2 update (an2_step's bookkeeping)
+GroupIdx group_next(GroupIdx n) {
+  if (n.count > 0) --n.count;
+  int *slot = n.b == 0 ? &n.s0 : &n.s1;
+  const int oldR = *slot;
+  *slot = n.iR;
+  n.iR = oldR;
+  n.iT = 1 - n.iT;
+  n.b = n.b == 0 ? 1 : 0;
+  return n;
+}
+
 // contexts whose steps can share launches: equal kernel parameters (grid,
 // box, solvent, closure, chi source, launch plans), AN2 options and state
 bool groupable(const rism_ctx *a, const rism_ctx *b) {
@@ -1042,7 +1054,8 @@ bool groupable(const rism_ctx *a, const rism_ctx *b) {
   return a->nx == b->nx && a->ny == b->ny && a->nz == b->nz && a->natv == b->natv &&
          a->box[0] == b->box[0] && a->box[1] == b->box[1] && a->box[2] == b->box[2] &&
          a->solv_hash == b->solv_hash && p.clos == q.clos && p.shell == q.shell &&
-         a->opts.an2_count == b->opts.an2_count && a->opts.an2_m == b->opts.an2_m &&
+         a->opts.an2_count == b->opts.an2_count && a->opts.an2_count <= 32 &&
+         a->opts.an2_m == b->opts.an2_m &&
          a->opts.an2_mp == b->opts.an2_mp && p.var_zf[0] == q.var_zf[0] &&
          p.var_zi[0] == q.var_zi[0] && p.var_y[0] == q.var_y[0] && a->prec == 0 &&
          b->prec == 0 && group_idx(a) == group_idx(b);
@@ -1055,12 +1068,10 @@ struct GroupRun {
   int istep = 0, maxstep = 0;
   double tol = 0.0;
   bool finished = false;
-  // device tables by state, in one allocation of kMaxStates tables; their
-  // host images stay alive until the group ends (asynchronous uploads)
-  static constexpr int kMaxStates = 32;
+  // device tables by state (one allocation, a table per state)
+  static constexpr int kMaxStates = 64;
   rism::GroupPtrs *dtab = nullptr;
   std::map<GroupIdx, int> tabs;
-  std::vector<std::vector<rism::GroupPtrs>> host;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   GroupIdx gix{};
@@ -1077,48 +1088,58 @@ struct GroupRun {
 
 // the device table of state ix: each member's buffers for one step
 int group_table(GroupRun &g, const GroupIdx &ix, rism::GroupPtrs **out) {
-  const size_t k = g.m.size();
   auto it = g.tabs.find(ix);
-  if (it != g.tabs.end()) {
-    *out = g.dtab + (size_t)it->second * k;
-    return RISM_OK;
-  }
-  const int slot = (int)g.host.size();
-  if (slot >= GroupRun::kMaxStates)
-    return rism::set_error(RISM_ERR_STATE, "too many accelerator states in a group");
-  g.host.emplace_back(k, rism::GroupPtrs{});
-  std::vector<rism::GroupPtrs> &h = g.host.back();
-  for (size_t i = 0; i < k; ++i) {
-    rism_ctx *c = g.m[i];
-    Level<double> &L = c->d;
-    rism::GroupPtrs &e = h[i];
-    e.t = L.T[ix.iT];
-    e.u = L.u;
-    e.fr = L.fr;
-    e.spec = L.spec;
-    e.fk = L.fk;
-    e.R = L.R[ix.iR];
-    e.R0 = L.R[ix.s0];
-    e.R1 = L.R[ix.s1];
-    e.partial = c->partial;
-    e.sums = c->sums;
-    e.scal = c->scal;
-    e.rms_hist = c->rms_hist;
-    e.done = c->dflag;
-    e.tnew = L.T[1 - ix.iT];
-    e.P0 = L.Pslot[0];
-    e.P1 = L.Pslot[1];
-    e.Pb = L.Pslot[ix.b];
-  }
-  rism::GroupPtrs *d = g.dtab + (size_t)slot * k;
-  g.tabs[ix] = slot;
-  // ordered on the group's stream before the launches that read it (never
-  // called under capture: group_graph makes the period's tables first)
-  HIPCHECK(hipMemcpyAsync(d, h.data(), sizeof(rism::GroupPtrs) * k, hipMemcpyHostToDevice,
-                          g.c0()->st));
-  *out = d;
+  if (it == g.tabs.end())
+    return rism::set_error(RISM_ERR_STATE, "group step in an unexpected accelerator state");
+  *out = g.dtab + (size_t)it->second * g.m.size();
   return RISM_OK;
 }
+
+// every state the group's steps pass through (the Picard steps, then the
+// period of the slot rotation), their tables built and uploaded at once
+int group_tables(GroupRun &g) {
+  const size_t k = g.m.size();
+  std::vector<rism::GroupPtrs> h;
+  GroupIdx ix = g.ix;
+  while (g.tabs.find(ix) == g.tabs.end()) {
+    const int slot = (int)g.tabs.size();
+    if (slot >= GroupRun::kMaxStates)
+      return rism::set_error(RISM_ERR_STATE, "too many accelerator states in a group");
+    g.tabs[ix] = slot;
+    for (size_t i = 0; i < k; ++i) {
+      rism_ctx *c = g.m[i];
+      Level<double> &L = c->d;
+      rism::GroupPtrs e{};
+      e.t = L.T[ix.iT];
+      e.u = L.u;
+      e.fr = L.fr;
+      e.spec = L.spec;
+      e.fk = L.fk;
+      e.R = L.R[ix.iR];
+      e.R0 = L.R[ix.s0];
+      e.R1 = L.R[ix.s1];
+      e.partial = c->partial;
+      e.sums = c->sums;
+      e.scal = c->scal;
+      e.rms_hist = c->rms_hist;
+      e.done = c->dflag;
+      e.tnew = L.T[1 - ix.iT];
+      e.P0 = L.Pslot[0];
+      e.P1 = L.Pslot[1];
+      e.Pb = L.Pslot[ix.b];
+      h.push_back(e);
+    }
+    ix = group_next(ix);
+  }
+  if (hipMalloc(&g.dtab, sizeof(rism::GroupPtrs) * h.size()) != hipSuccess)
+    return rism::set_error(RISM_ERR_NOMEM, "hipMalloc of the group tables failed");
+  hipStream_t st = g.c0()->st;
+  HIPCHECK(hipMemcpyAsync(g.dtab, h.data(), sizeof(rism::GroupPtrs) * h.size(),
+                          hipMemcpyHostToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return RISM_OK;
+}
+
 
 // one step (calc_step) and one AN2 update (an2_step) of every member at the
 // group's state g.ix, which then advances as an2_step advances a context's
@@ -1157,15 +1178,7 @@ int group_step(GroupRun &g) {
                                   L.Pslot[1], L.Pslot[ix.b], c->scal, c->opts.an2_m, nn,
                                   (const int *)nullptr, st, tab, P.ngrp));
   }
-  // an2_step's bookkeeping
-  GroupIdx &n = g.ix;
-  if (n.count > 0) --n.count;
-  int *slot = n.b == 0 ? &n.s0 : &n.s1;
-  const int oldR = *slot;
-  *slot = n.iR;
-  n.iR = oldR;
-  n.iT = 1 - n.iT;
-  n.b = n.b == 0 ? 1 : 0;
+  g.ix = group_next(g.ix);
   return RISM_OK;
 }
 
@@ -1177,24 +1190,7 @@ int group_graph(GroupRun &g) {
   if (g.graph) (void)hipGraphDestroy(g.graph);
   g.gexec = nullptr;
   g.graph = nullptr;
-  // the tables first: their copies must not be captured
-  GroupIdx probe = g.ix;
-  {
-    const GroupIdx keep = g.ix;
-    for (int k = 0; k < kGraphSteps; ++k) {
-      rism::GroupPtrs *t = nullptr;
-      int rc = group_table(g, g.ix, &t);
-      if (rc) return rc;
-      GroupIdx &n = g.ix;  // advance as group_step does
-      int *slot = n.b == 0 ? &n.s0 : &n.s1;
-      const int oldR = *slot;
-      *slot = n.iR;
-      n.iR = oldR;
-      n.iT = 1 - n.iT;
-      n.b = n.b == 0 ? 1 : 0;
-    }
-    g.ix = keep;
-  }
+  const GroupIdx probe = g.ix;
   hipStream_t st = g.c0()->st;
   HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int rc = RISM_OK;
@@ -1218,9 +1214,6 @@ int group_begin(GroupRun &g, int maxstep, double tol) {
   g.tol = tol;
   g.istep = 0;
   g.ix = group_idx(g.c0());
-  if (hipMalloc(&g.dtab, sizeof(rism::GroupPtrs) * GroupRun::kMaxStates * g.m.size()) !=
-      hipSuccess)
-    return rism::set_error(RISM_ERR_NOMEM, "hipMalloc of the group tables failed");
   for (size_t i = 0; i < g.m.size(); ++i) {
     int rc = async_begin(*g.runs[i], g.m[i], maxstep, tol);
     if (rc) return rc;
@@ -1229,7 +1222,7 @@ int group_begin(GroupRun &g, int maxstep, double tol) {
   for (rism_ctx *c : g.m)
     if (hipStreamSynchronize(c->st) != hipSuccess)
       return rism::set_error(RISM_ERR_HIP, "hipStreamSynchronize failed");
-  return RISM_OK;
+  return group_tables(g);  // after async_begin: the tables hold rms_hist
 }
 
 // enqueues up to `chunk` steps of the group and the read-back of every
@@ -1250,13 +1243,7 @@ int group_enqueue(GroupRun &g, int chunk) {
       HIPCHECK(hipGraphLaunch(g.gexec, c->st));
       for (int j = 0; j < kGraphSteps; ++j) {
         log_step();
-        GroupIdx &n = g.ix;
-        int *slot = n.b == 0 ? &n.s0 : &n.s1;
-        const int oldR = *slot;
-        *slot = n.iR;
-        n.iR = oldR;
-        n.iT = 1 - n.iT;
-        n.b = n.b == 0 ? 1 : 0;
+        g.ix = group_next(g.ix);
       }
       k += kGraphSteps;
       continue;
@@ -2008,6 +1995,16 @@ int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads
           break;
         }
       if (!put) groups.push_back({i});
+    }
+  }
+  // a group of one runs the per-context sync-free path (its kernels read
+  // their pointers from their arguments rather than from a table)
+  for (size_t q = 0; q < groups.size();) {
+    if (groups[q].size() == 1) {
+      single.push_back(groups[q][0]);
+      groups.erase(groups.begin() + q);
+    } else {
+      ++q;
     }
   }
   std::vector<std::unique_ptr<GroupRun>> gruns(groups.size());
