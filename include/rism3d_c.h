@@ -174,13 +174,14 @@ int rism_anderson(rism_ctx *ctx);
 /* ---- batches of independent solves (no counterpart in the reference,
  * which runs one solute per process) ------------------------------------ */
 /* Runs rism_solve on n distinct contexts at once from `threads` host
- * threads, each context on its own stream.  FP64 AN2 contexts (precision 0,
- * anderson_m 0, verbose 0) have their steps enqueued in chunks without a
- * host round trip per step (a device flag stops a context's kernels at
- * convergence); the others (mixed precision, Anderson(m), verbose) run
- * rism_solve's step-by-step loop on their worker thread.  The results are
- * those of n separate rism_solve calls.  status/steps/rms: n entries each
- * (may be NULL). */
+ * threads.  FP64 contexts (precision 0, verbose 0) have their steps
+ * enqueued in chunks without a host round trip per step (a device flag
+ * stops a context's kernels at convergence); AN2 contexts of equal grid,
+ * box, solvent, options and accelerator state are stepped in groups of up
+ * to 16 that share one launch per kernel, on the first member's stream.
+ * Mixed-precision and verbose contexts run rism_solve's step-by-step loop
+ * on their worker thread.  The results are those of n separate rism_solve
+ * calls.  status/steps/rms: n entries each (may be NULL). */
 int rism_solve_batch(rism_ctx **ctx, int n, int maxstep, double tol, int threads,
                      int *status, int *steps, double *rms);
 /* Potentials, batched solve and thermodynamics for n contexts (rism_run for
