@@ -253,11 +253,13 @@ __global__ void __launch_bounds__(B *TZ)
     }
   };
   auto load = [&](int) -> C { return C{0, 0}; };
-  if constexpr (HM == 0 && sizeof(Re) == 8) {
-    // FP64 AN2 and Picard steps and the plain inverse: the loads of two
-    // outputs at a time are issued together (prep), then the residuals
-    // formed (put): 5.7 -> 5.5 ms at 512^3, 4 sites (the FP32 tile, at 64
-    // VGPRs, measured 3 % slower this way and keeps the plain chain)
+  if constexpr (HM == 0 && (sizeof(Re) == 8 || KIND != ZK_DIFF)) {
+    // AN2 and Picard steps and the plain inverse: the loads of two outputs at
+    // a time are issued together (prep), then the residuals formed (put):
+    // FP64 5.7 -> 5.5 ms at 512^3, 4 sites.  In the 64-VGPR FP32 tile this
+    // form also fits without spilling, where the plain chain spills one
+    // VGPR per output (2 GB of scratch traffic a launch at 512^3); the FP32
+    // difference form keeps the plain chain (the staged one spills more)
     struct Store {
       using staged = void;
       enum { group = 2 };
