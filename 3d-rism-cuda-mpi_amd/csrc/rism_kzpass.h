@@ -128,10 +128,12 @@ struct ZinvSums {
 // of LDS, 1024 threads) leaves room for a second workgroup per CU once the
 // kernel fits in 64 VGPRs, and the FP64 Anderson(m) tile (512 threads, 74 KB)
 // once it fits in 128; with one workgroup per CU the load, transform and
-// residual phases of a tile do not overlap.  The compiler spills 36-116 B
-// per lane to meet these limits, far cheaper than the lost overlap (512^3,
-// 4 sites: FP32 4.7 -> 3.2 ms, FP64 Anderson(10) 24.4 -> 14.9 ms).  The FP64
-// AN2 tile needs 148 KB of LDS and stays at one workgroup per CU.
+// residual phases of a tile do not overlap (512^3, 4 sites: FP32 4.7 ->
+// 3.2 ms, FP64 Anderson(10) 24.4 -> 14.9 ms, even when the limit made the
+// compiler spill; the FP32 AN2 and Picard tiles and the FP64 Anderson(m)
+// tile now fit without spills; the FP32 difference-form and Anderson(m)
+// tiles spill 8-112 B per lane).  The FP64 AN2 tile needs 148 KB of LDS and stays at
+// one workgroup per CU.
 template <int HM, class Re>
 constexpr int zinv_waves_per_eu() {
   return sizeof(Re) == 4 ? 8 : (HM ? 4 : 1);
