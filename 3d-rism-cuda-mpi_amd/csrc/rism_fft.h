@@ -194,6 +194,14 @@ constexpr RadixList make_radices(int L, int maxr = 8) {
   return rl;
 }
 
+// whether the last stage of the plan of length L with T threads per column
+// has exactly two butterflies per thread (a mirror-block store can be used)
+template <int L, int T, int MAXR>
+constexpr bool mirror_last_stage() {
+  constexpr RadixList rl = make_radices(L, MAXR);
+  return rl.n > 1 && L / rl.r[rl.n - 1] == 2 * T;
+}
+
 constexpr int max_radix(int L, int maxr = 8) {
   RadixList rl = make_radices(L, maxr);
   int mx = 1;
@@ -227,6 +235,18 @@ struct is_staged : std::false_type {};
 template <class T>
 struct is_staged<T, std::void_t<typename T::staged>> : std::true_type {};
 
+// A mirror-block Store (with a member type `mirror_block`) takes the last
+// stage's outputs in registers: that stage runs butterflies j and NB-1-j in
+// thread j (it needs exactly two butterflies per thread), whose outputs are
+// the elements j + r NS and (NS-1-j) + r NS, so element m and element L-1-m
+// are in the same thread; it calls block(j, NB-1-j, v) once with
+// v[0][r], v[1][r] the two butterflies' outputs (no LDS writes and no
+// barrier).  mirror_last_stage<L, T, MAXR>() says whether a plan allows it.
+template <class T, class = void>
+struct is_mirror_block : std::false_type {};
+template <class T>
+struct is_mirror_block<T, std::void_t<typename T::mirror_block>> : std::true_type {};
+
 // w[r] = w[1]^r for r = 2 .. R-1, each by one complex product of lower
 // powers (at most three products deep, so within a few ulps of the
 // correctly rounded table value).  One table load per butterfly instead of
@@ -251,6 +271,8 @@ __device__ __forceinline__ void stockham_stage(cx<Re> *lds, int tid,
                                                Load &load, Store &store) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
+  constexpr bool MB = LAST && is_mirror_block<Store>::value;
+  static_assert(!MB || NB == 2 * T, "a mirror-block store needs two butterflies per thread");
   cx<Re> v[PER][R];
   if constexpr (FIRST && is_staged<Load>::value) {
     // Load::group elements are fetched together (bounded by the registers
@@ -275,7 +297,7 @@ __device__ __forceinline__ void stockham_stage(cx<Re> *lds, int tid,
   }
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
-    const int j = tid + b * T;
+    const int j = MB ? (b == 0 ? tid : NB - 1 - tid) : tid + b * T;
     if ((NB % T == 0) || j < NB) {
       if constexpr (!(FIRST && is_staged<Load>::value)) {
 #pragma unroll
@@ -298,38 +320,42 @@ __device__ __forceinline__ void stockham_stage(cx<Re> *lds, int tid,
       Dft<R, S>::run(v[b]);
     }
   }
-  if constexpr (!FIRST) __syncthreads();
+  if constexpr (MB) {
+    store.block(tid, NB - 1 - tid, v);
+  } else {
+    if constexpr (!FIRST) __syncthreads();
 #pragma unroll
-  for (int b = 0; b < PER; ++b) {
-    const int j = tid + b * T;
-    if ((NB % T == 0) || j < NB) {
-      const int k = j % NS;
-      const int idxD = (j / NS) * NS * R + k;
-      if constexpr (LAST && is_staged<Store>::value) {
-        constexpr int G = (int)Store::group;
+    for (int b = 0; b < PER; ++b) {
+      const int j = tid + b * T;
+      if ((NB % T == 0) || j < NB) {
+        const int k = j % NS;
+        const int idxD = (j / NS) * NS * R + k;
+        if constexpr (LAST && is_staged<Store>::value) {
+          constexpr int G = (int)Store::group;
 #pragma unroll
-        for (int r0 = 0; r0 < R; r0 += G) {
-          typename Store::aux_t aux[G];
+          for (int r0 = 0; r0 < R; r0 += G) {
+            typename Store::aux_t aux[G];
 #pragma unroll
-          for (int g = 0; g < G; ++g)
-            if (r0 + g < R) aux[g] = store.prep(idxD + (r0 + g) * NS);
+            for (int g = 0; g < G; ++g)
+              if (r0 + g < R) aux[g] = store.prep(idxD + (r0 + g) * NS);
 #pragma unroll
-          for (int g = 0; g < G; ++g)
-            if (r0 + g < R) store.put(idxD + (r0 + g) * NS, v[b][r0 + g], aux[g]);
-        }
-      } else {
+            for (int g = 0; g < G; ++g)
+              if (r0 + g < R) store.put(idxD + (r0 + g) * NS, v[b][r0 + g], aux[g]);
+          }
+        } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int idx = idxD + r * NS;
-          if constexpr (LAST)
-            store(idx, v[b][r]);
-          else
-            lds[lpad(idx)] = v[b][r];
+          for (int r = 0; r < R; ++r) {
+            const int idx = idxD + r * NS;
+            if constexpr (LAST)
+              store(idx, v[b][r]);
+            else
+              lds[lpad(idx)] = v[b][r];
+          }
         }
       }
     }
+    if constexpr (!LAST) __syncthreads();
   }
-  if constexpr (!LAST) __syncthreads();
 }
 
 template <int L, int S, int T, int STAGE, int NS, bool FROM_GLOBAL, bool TO_GLOBAL, int MAXR,

@@ -96,19 +96,51 @@ __global__ void __launch_bounds__(B *TZ)
     if (!ok) return C{0, 0};
     return cmul(C{cval(2 * p), cval(2 * p + 1)}, cmul(zprej, zpre[p - j]));
   };
-  auto store = [&](int, C) {};
-  fft_column<M, -1, true, false, TZ, MAXR>(col, j, twM, load, store);
-
   C *out = spec + ((size_t)v * M * P.ny + y) * P.nx + x;
   const size_t mstride = (size_t)P.ny * P.nx;
-  // zpost[m] = e^{-i pi (2m+1)/nz} = zpost[j] zpre[m - j]
-  const C zpostj = zpost[j];
-  for (int m = j; m < M; m += T) {
-    const C a = col[lpad(m)];
-    const C b = conj(col[lpad(M - 1 - m)]);
-    const C E = (Re)0.5 * (a + b);
-    const C O = mul_si<-1>((Re)0.5 * (a - b));
-    if (ok) out[(size_t)m * mstride] = E + cmul(cmul(zpostj, zpre[m - j]), O);
+  if constexpr (mirror_last_stage<M, TZ, MAXR>()) {
+    // the last stage leaves Z(m) and Z(M-1-m) in the same thread (butterflies
+    // j and NS-1-j): the untangle G(m) = E(m) + w_m O(m) is formed from the
+    // registers and written out (no LDS write, read and barrier)
+    constexpr RadixList rl = make_radices(M, MAXR);
+    constexpr int RL = rl.r[rl.n - 1], NS = M / RL;
+    struct Untangle {
+      using mirror_block = void;
+      C *out;
+      size_t mstride;
+      const C *zpre;
+      C zp0, zp1;  // zpost at the two butterflies' first elements
+      bool ok;
+      __device__ __forceinline__ void half(int m, C a, C b, C zp, C zr) {
+        const C E = (Re)0.5 * (a + conj(b));
+        const C O = mul_si<-1>((Re)0.5 * (a - conj(b)));
+        // zpost[m0 + c] = zpost[m0] zpre[c]
+        out[(size_t)m * mstride] = E + cmul(cmul(zp, zr), O);
+      }
+      __device__ __forceinline__ void block(int j0, int j1, C (&w)[2][RL]) {
+        if (!ok) return;
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const C zr = zpre[r * NS];
+          // element j0 + r NS pairs with M-1-(j0 + r NS) = j1 + (RL-1-r) NS
+          half(j0 + r * NS, w[0][r], w[1][RL - 1 - r], zp0, zr);
+          half(j1 + r * NS, w[1][r], w[0][RL - 1 - r], zp1, zr);
+        }
+      }
+    } store{out, mstride, zpre, zpost[j], zpost[NS - 1 - j], ok};
+    fft_column<M, -1, true, true, TZ, MAXR>(col, j, twM, load, store);
+  } else {
+    auto store = [&](int, C) {};
+    fft_column<M, -1, true, false, TZ, MAXR>(col, j, twM, load, store);
+    // zpost[m] = e^{-i pi (2m+1)/nz} = zpost[j] zpre[m - j]
+    const C zpostj = zpost[j];
+    for (int m = j; m < M; m += T) {
+      const C a = col[lpad(m)];
+      const C b = conj(col[lpad(M - 1 - m)]);
+      const C E = (Re)0.5 * (a + b);
+      const C O = mul_si<-1>((Re)0.5 * (a - b));
+      if (ok) out[(size_t)m * mstride] = E + cmul(cmul(zpostj, zpre[m - j]), O);
+    }
   }
 }
 
