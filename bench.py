@@ -355,6 +355,10 @@ def phase_mixed(args, budget, R, ctx):
     R["out"]["mixed"] = {
         "s_per_solve": wall / nt, "solves_timed": nt, "warmup": nw,
         "speedup_vs_fp64": s64 / (wall / nt),
+        # per iteration (iterate phase over steps): independent of how many
+        # steps each solve's trajectory takes
+        "iteration_speedup_vs_fp64": ((ctx["tm"]["iterate_ms"] / ctx["tm"]["steps"])
+                                      / (tmm["iterate_ms"] / tmm["steps"])),
         "iterations": tmm["steps"], "iterations_fp32": tmm["steps_fp32"],
         "status": tmm["status"], "setup_ms": tmm["setup_ms"], "iterate_ms": tmm["iterate_ms"],
         "SFE_SC": thm["SFE_SC"], "dSFE_SC_J_per_mol": dmu, "dSFE_SC_kcal_per_mol": dmu / KCAL,
