@@ -111,7 +111,7 @@ typedef struct {
                          /* steps for blocks of difference-form FP32 steps    */
                          /* between FP64 steps (0: 1e-3); it also leaves when */
                          /* the FP32 rms stops improving                      */
-  int mixed_block;       /* FP32 steps per block (0: 24)                      */
+  int mixed_block;       /* FP32 steps per block (0: 48)                      */
 } rism_opts;
 
 /* Thermodynamics of the converged state (output.cu:23-46, output_xmu.cu). */
