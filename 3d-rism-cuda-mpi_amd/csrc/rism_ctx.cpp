@@ -1421,7 +1421,7 @@ int tune_launch(rism_ctx *c, int pass) {
 
 // Picks the plan of each tuned pass for level Re (the cache first, else by
 // timing) and records it in c->P.  The level's fields hold no data yet:
-// they are cleared first so every plan runs on the same (zero) values.
+// they are set first so every plan runs on the same values (zero, u = 0.5).
 template <class Re>
 int tune_level(rism_ctx *c) {
   const int f = sizeof(Re) == 4 ? 1 : 0;
@@ -1455,7 +1455,13 @@ int tune_level(rism_ctx *c) {
     }
     if (!cleared) {
       HIPCHECK(hipMemsetAsync(L.T[0], 0, sizeof(Re) * nn, c->st));
-      HIPCHECK(hipMemsetAsync(L.u, 0, sizeof(Re) * nn, c->st));
+      // u = 0.5 (the 32-bit word of 0.5f repeated: 0.5 + 2^-30 in FP64), so
+      // that t - u < 0 and the closure takes its exponential branch, as it
+      // does over much of a real grid: with u = 0 the KH and PSE closures
+      // take the linear branch, the z passes skip their exponentials, and a
+      // plan that cannot hide them (the 512^3 forward pass: 3.37 ms on zeros,
+      // 3.65 ms on a solve's fields) is timed too fast
+      HIPCHECK(hipMemsetD32Async((hipDeviceptr_t)L.u, 0x3F000000u, sizeof(Re) * nn / 4, c->st));
       HIPCHECK(hipMemsetAsync(L.fr, 0, sizeof(Re) * c->N, c->st));
       HIPCHECK(hipMemsetAsync(L.spec, 0, sizeof(cx<Re>) * nn / 2, c->st));
       for (int i = 0; i < 3; ++i)
@@ -1487,6 +1493,8 @@ int tune_level(rism_ctx *c) {
     std::lock_guard<std::mutex> g(tune::mu);
     if (tune::table.emplace(key, rec).second) tune::save_locked(key, rec);
   }
+  // u back to zero after the timing (the other fields stay cleared)
+  if (cleared) HIPCHECK(hipMemsetAsync(L.u, 0, sizeof(Re) * nn, c->st));
   return RISM_OK;
 }
 
