@@ -165,6 +165,28 @@ def test_lambda_switches(rism, oracle, gpu, case24):
         g.close()
 
 
+@pytest.mark.parametrize("solvent", ["h2o_vv.xvk", "tip3p_nacl02_drism.xvk"])
+@pytest.mark.parametrize("ad,lam", [(0, 1.0), (1, 0.8)])
+def test_atoms_on_grid_points(rism, oracle, gpu, golden, solvent, ad, lam):
+    """Atoms exactly on grid points: the s^2 >= 4e-6 floor (cal_LJ.cu:11-12)
+    and the r < 1e-5 Coulomb guard bind there, so the potential kernel's
+    clamped pair path runs, for 2 and 4 sites, with Coulomb (ad 0) and
+    without it (ad 1: -l).  u reaches ~1e13 at those points."""
+    s, d = _solvent(oracle, golden, solvent)
+    grid, box = (32, 32, 32), (16.0, 16.0, 16.0)  # dr = 0.5: grid points at k/2 A
+    xyz = np.array([[0.0, 0.0, 0.0], [1.5, -1.0, 2.5], [-2.0, 2.0, 0.0], [1.23, -0.77, 2.1]])
+    sol = oracle.Solute(np.array([3.40, 2.96, 3.25, 2.50]), np.array([0.086, 0.21, 0.17, 0.015]),
+                        np.array([0.4, -0.5, -0.2, 0.3]), xyz)
+    u, fr, fk = oracle.potentials(grid, box, s, sol, lam=lam, adswitch=ad)
+    assert np.abs(u).max() > 1e12  # the floor was reached
+    g = rism.RISM3D(grid, box, d, _solute(sol), ad_switch=ad, lam=lam)
+    g.initialize()
+    assert np.allclose(g.fields(rism.FIELD_U).reshape(-1), u, rtol=1e-12, atol=1e-12)
+    assert np.allclose(g.get(rism.FIELD_FR), fr, rtol=1e-12,
+                       atol=1e-13 * max(np.abs(fr).max(), 1.0))
+    g.close()
+
+
 def test_krange_error(rism, oracle, gpu, golden):
     """|k|max beyond the chi table -> RISM_ERR_KRANGE (solvent_spline.cu:17-20)."""
     s, d = _solvent(oracle, golden)
