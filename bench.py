@@ -503,8 +503,28 @@ def phase_batch(args, budget, R, nsol=256, prec=0, stepwise=True):
             g.close()
         res["sequential_stepwise_s"] = t_step
         res["speedup_vs_sequential_stepwise"] = t_step / t_batch
+    res["autotune"] = tune_digest(rism, set(tuple(c[0]) for c in cases))
     log(f"batch: {nsol / t_batch:.1f} solves/s, {t_seq / t_batch:.2f}x sequential")
     return res
+
+
+def tune_digest(rism, grids):
+    """The autotuner's choices over a set of grids, per tuned pass and bulk
+    type: how many grids it timed, how many it moved off the default plan,
+    and the mean and largest gain of the chosen plan over the default."""
+    agg = {}
+    for e in rism.tune_report():
+        if tuple(e["grid"]) not in grids:
+            continue
+        ms = e["ms"]
+        gain = ms[0] / ms[e["choice"]]
+        a = agg.setdefault((e["pass"], e["dtype"]), [0, 0, 0.0, 1.0])
+        a[0] += 1
+        a[1] += 1 if e["choice"] != 0 else 0
+        a[2] += gain
+        a[3] = max(a[3], gain)
+    return [{"pass": k[0], "dtype": k[1], "grids": v[0], "off_default": v[1],
+             "gain_mean": v[2] / v[0], "gain_max": v[3]} for k, v in sorted(agg.items())]
 
 
 # ---------------------------------------------------------------------------
@@ -786,6 +806,7 @@ def bench_medium(args):
         "dSFE_kcal_per_mol_max": float(dmu.max()) if len(dmu) else None,
         "dSFE_kcal_per_mol_p50": float(np.percentile(dmu, 50)) if len(dmu) else None,
         "within_0.05_kcal_per_mol": int((dmu <= 0.05).sum()),
+        "autotune": tune_digest(rism, set(tuple(c[0]) for c in cases)),
     }
     print(json.dumps(res), flush=True)
 
