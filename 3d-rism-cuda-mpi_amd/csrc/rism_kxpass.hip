@@ -150,8 +150,11 @@ __device__ __forceinline__ void oz_points_reg(cx<Re> *lds, int CS, const int (&o
   }
 }
 
-// One workgroup per (m, y) row, all sites: natv columns of nx complex in LDS,
-// natv * T threads.  SHELL selects the chi source at compile time (the
+// One workgroup per m and pair of mirror rows (y, ny-1-y), all sites: natv
+// columns of nx complex in LDS, natv * T threads, the two rows one after the
+// other.  The rows of a pair have the same |ky| and so the same chi rows
+// (the shell index and |k| depend on ky^2 only): the register path loads
+// them once for both rows.  SHELL selects the chi source at compile time (the
 // cubic-box shell table or the spline at |k|) and NVMAX bounds the site
 // count, so each instantiation carries only the registers it needs.
 // dynamic LDS of the x passes, shared by all instantiations; 16-byte
@@ -178,11 +181,11 @@ __global__ void __launch_bounds__(512)
   const int tid = threadIdx.x;
   const int nv = P.natv;
   const int v = tid / T, j = tid % T;
-  const int y = blockIdx.x, m = blockIdx.y;
+  const int y0 = blockIdx.x, m = blockIdx.y;
+  const int y1 = P.ny - 1 - y0;          // the mirror row (y0 itself for the middle row of odd ny)
+  const int nrows = y1 > y0 ? 2 : 1;
   const int M = P.nz / 2;
   const size_t rowstride = (size_t)M * P.ny * L;  // site stride in spec
-  const size_t rowoff = ((size_t)m * P.ny + y) * L;
-  C *row = spec + (size_t)v * rowstride + rowoff;
   C *mylds = lds + v * CS;
   // the spectrum row is read once and written once: non-temporal accesses
   // leave the caches to the chi table, fk and the twiddles.  The element
@@ -190,27 +193,16 @@ __global__ void __launch_bounds__(512)
   // every thread, and the pre-twiddle is an exponential, so prex[j + c] =
   // prex[j] prex[c]: one per-thread load, the rest uniform (scalar) loads.
   const C prej = prex[j];
-  auto load = [&](int i) -> C {
-    const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
-    return cmul(a, cmul(prej, prex[i - j]));
-  };
-  auto store = [&](int i, C vv) {
-    const C o = cmulc(vv, cmul(prej, prex[i - j]));
-    __builtin_nontemporal_store(o.x, &row[i].x);
-    __builtin_nontemporal_store(o.y, &row[i].y);
-  };
   auto nostore = [&](int, C) {};
   const int nthr = nv * T;
   const int nv2 = nv * nv;
-  // the difference-form step (mode 3): the long-range term cancels, fk = 0
-  const C *fkr = (P.mode == 3) ? nullptr : fkraw + rowoff;
-  const long oy = 2 * y - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates
+  const long oy = 2 * y0 - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates (|oy| of both rows)
   // at most one (kx, -kx) pair per thread and at most 4 sites: fetch this
-  // thread's chi row and fk pair before the forward transform
+  // thread's chi row (shared by the two rows) and the fk pair of each row
+  // before its forward transform
   if constexpr (SHELL && NVMAX == 4) {
     if (L / 2 <= nthr) {
       Re chr[4][4];
-      C fkp[2] = {C{0, 0}, C{0, 0}};
       const bool mine = tid < L / 2;
       if (mine) {
         const long ox = 2 * tid - L + 1;
@@ -220,46 +212,89 @@ __global__ void __launch_bounds__(512)
         for (int v = 0; v < 4; ++v)
 #pragma unroll
           for (int w = 0; w < 4; ++w) chr[v][w] = (v < nv && w < nv) ? rowp[v * nv + w] : (Re)0;
-        if (fkr) {
+      }
+#pragma unroll 1
+      for (int r = 0; r < nrows; ++r) {
+        // an opaque zero: the table loads of the row's transforms stay in the
+        // loop (hoisted out of it, they held ~100 more VGPRs)
+        int zero = 0;
+        asm volatile("" : "+s"(zero));
+        const C *tw = twL + zero, *px = prex + zero;
+        const size_t rowoff = ((size_t)m * P.ny + (r ? y1 : y0)) * L;
+        C *row = spec + (size_t)v * rowstride + rowoff;
+        auto load = [&](int i) -> C {
+          const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
+          return cmul(a, cmul(prej, px[i - j]));
+        };
+        auto store = [&](int i, C vv) {
+          const C o = cmulc(vv, cmul(prej, px[i - j]));
+          __builtin_nontemporal_store(o.x, &row[i].x);
+          __builtin_nontemporal_store(o.y, &row[i].y);
+        };
+        // the difference-form step (mode 3): the long-range term cancels, fk = 0
+        const C *fkr = (P.mode == 3) ? nullptr : fkraw + rowoff;
+        C fkp[2] = {C{0, 0}, C{0, 0}};
+        if (mine && fkr) {
           fkp[0] = fkr[tid];
           fkp[1] = fkr[L - 1 - tid];
         }
-      }
-      fft_column<L, -1, true, false>(mylds, j, twL, load, nostore);
-      if (mine) {
-        const int off[2] = {lpad(tid), lpad(L - 1 - tid)};
-        switch (nv) {
-          case 1: oz_points_reg<1>(lds, CS, off, fkp, P, chr); break;
-          case 2: oz_points_reg<2>(lds, CS, off, fkp, P, chr); break;
-          case 3: oz_points_reg<3>(lds, CS, off, fkp, P, chr); break;
-          default: oz_points_reg<4>(lds, CS, off, fkp, P, chr); break;
+        if (r) __syncthreads();  // the first row's inverse stage has read its LDS
+        fft_column<L, -1, true, false>(mylds, j, tw, load, nostore);
+        if (mine) {
+          const int off[2] = {lpad(tid), lpad(L - 1 - tid)};
+          switch (nv) {
+            case 1: oz_points_reg<1>(lds, CS, off, fkp, P, chr); break;
+            case 2: oz_points_reg<2>(lds, CS, off, fkp, P, chr); break;
+            case 3: oz_points_reg<3>(lds, CS, off, fkp, P, chr); break;
+            default: oz_points_reg<4>(lds, CS, off, fkp, P, chr); break;
+          }
         }
+        __syncthreads();
+        fft_column<L, +1, false, true>(mylds, j, tw, load, store);
       }
-      __syncthreads();
-      fft_column<L, +1, false, true>(mylds, j, twL, load, store);
       return;
     }
   }
-  fft_column<L, -1, true, false>(mylds, j, twL, load, nostore);
+#pragma unroll 1
+  for (int r = 0; r < nrows; ++r) {
+    int zero = 0;  // an opaque zero, as above
+    asm volatile("" : "+s"(zero));
+    const C *tw = twL + zero, *px = prex + zero;
+    const int y = r ? y1 : y0;
+    const size_t rowoff = ((size_t)m * P.ny + y) * L;
+    C *row = spec + (size_t)v * rowstride + rowoff;
+    auto load = [&](int i) -> C {
+      const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
+      return cmul(a, cmul(prej, px[i - j]));
+    };
+    auto store = [&](int i, C vv) {
+      const C o = cmulc(vv, cmul(prej, px[i - j]));
+      __builtin_nontemporal_store(o.x, &row[i].x);
+      __builtin_nontemporal_store(o.y, &row[i].y);
+    };
+    const C *fkr = (P.mode == 3) ? nullptr : fkraw + rowoff;
+    if (r) __syncthreads();  // the first row's inverse stage has read its LDS
+    fft_column<L, -1, true, false>(mylds, j, tw, load, nostore);
 
-  // OZ at every k of the row, the mirror points (kx, -kx) together
-  for (int jx = tid; jx < L / 2; jx += nthr) {
-    if constexpr (SHELL) {
-      const long ox = 2 * jx - L + 1;
-      const long sidx = (ox * ox + oy * oy + oz * oz - 3) >> 3;  // |k|^2 = (pi/L)^2 (8 sidx + 3)
-      oz_row<NVMAX>(lds, CS, jx, L, fkr, P, ChiTable<Re>{chitab + sidx * nv2});
-    } else {
-      const double ky = P.kfac[1] * ((double)y - (double)P.ny / 2.0 + 0.5);
-      const double kz = P.kfac[2] * ((double)(m + P.nz / 2) - (double)P.nz / 2.0 + 0.5);
-      const double kx = P.kfac[0] * ((double)jx - (double)L / 2.0 + 0.5);
-      const double kk = sqrt(kx * kx + ky * ky + kz * kz);
-      const int ii = spline_interval(knots, P.np, P.x0, P.idk, kk);
-      oz_row<NVMAX>(lds, CS, jx, L, fkr, P,
-                    ChiSpline{coef + (size_t)ii * 4 * nv2, kk - knots[ii], nv2});
+    // OZ at every k of the row, the mirror points (kx, -kx) together
+    for (int jx = tid; jx < L / 2; jx += nthr) {
+      if constexpr (SHELL) {
+        const long ox = 2 * jx - L + 1;
+        const long sidx = (ox * ox + oy * oy + oz * oz - 3) >> 3;  // |k|^2 = (pi/L)^2 (8 sidx + 3)
+        oz_row<NVMAX>(lds, CS, jx, L, fkr, P, ChiTable<Re>{chitab + sidx * nv2});
+      } else {
+        const double ky = P.kfac[1] * ((double)y - (double)P.ny / 2.0 + 0.5);
+        const double kz = P.kfac[2] * ((double)(m + P.nz / 2) - (double)P.nz / 2.0 + 0.5);
+        const double kx = P.kfac[0] * ((double)jx - (double)L / 2.0 + 0.5);
+        const double kk = sqrt(kx * kx + ky * ky + kz * kz);
+        const int ii = spline_interval(knots, P.np, P.x0, P.idk, kk);
+        oz_row<NVMAX>(lds, CS, jx, L, fkr, P,
+                      ChiSpline{coef + (size_t)ii * 4 * nv2, kk - knots[ii], nv2});
+      }
     }
+    __syncthreads();
+    fft_column<L, +1, false, true>(mylds, j, tw, load, store);
   }
-  __syncthreads();
-  fft_column<L, +1, false, true>(mylds, j, twL, load, store);
 }
 
 // Plain transform along x of the half spectrum (kernel-level entry points):
@@ -297,6 +332,8 @@ static void launch_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const
                        const cx<Re> *prex, const double *knots, const double *coef,
                        const Re *chitab, hipStream_t st) {
   dim3 grid(P.ny, P.nz / 2, P.grp ? P.ngrp : 1);
+  // the OZ pass takes the rows in mirror pairs (y, ny-1-y)
+  const dim3 pgrid((P.ny + 1) / 2, P.nz / 2, P.grp ? P.ngrp : 1);
   const int threads = P.natv * FftShape<L>::T;
   const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cx<Re>);
   if constexpr (sizeof(Re) == sizeof(double)) {
@@ -314,7 +351,7 @@ static void launch_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const
     if (lds > 65536)
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kern, grid, dim3(threads), lds, st, spec, fkraw, P, twL, prex, knots, coef,
+    hipLaunchKernelGGL(kern, pgrid, dim3(threads), lds, st, spec, fkraw, P, twL, prex, knots, coef,
                        chitab);
   };
   if (P.natv <= 4)
