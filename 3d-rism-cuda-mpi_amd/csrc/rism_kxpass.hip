@@ -180,13 +180,23 @@ __global__ void __launch_bounds__(512)
   } else if (P.done && *P.done) return;
   const int tid = threadIdx.x;
   const int nv = P.natv;
-  const int v = tid / T, j = tid % T;
-  const int y0 = blockIdx.x, m = blockIdx.y;
-  const int y1 = P.ny - 1 - y0;          // the mirror row (y0 itself for the middle row of odd ny)
-  const int nrows = y1 > y0 ? 2 : 1;
+  // row slots: a workgroup of short rows (T < 64) holds several row pairs
+  // side by side (one wave at least), each slot with its own natv LDS
+  // columns; with T >= 64 a workgroup is one slot, and the row indices stay
+  // workgroup-uniform (scalar branches, the code of one row per workgroup)
+  constexpr bool kSlots = T < 64;
+  const int nthr = nv * T;  // threads of one row slot
+  const int slot = kSlots ? tid / nthr : 0, tl = kSlots ? tid % nthr : tid;
+  const int v = tl / T, j = tl % T;
+  const int y0 = kSlots ? blockIdx.x * (int)(blockDim.x / nthr) + slot : (int)blockIdx.x;
+  const int m = blockIdx.y;
+  const bool live = !kSlots || y0 < (P.ny + 1) / 2;  // the last workgroup's slots may be spare
+  const int y1 = P.ny - 1 - y0;                       // the mirror row
+  const bool pair = live && y1 > y0;                  // not for the middle row of odd ny
   const int M = P.nz / 2;
   const size_t rowstride = (size_t)M * P.ny * L;  // site stride in spec
-  C *mylds = lds + v * CS;
+  C *slds = lds + slot * nv * CS;
+  C *mylds = slds + v * CS;
   // the spectrum row is read once and written once: non-temporal accesses
   // leave the caches to the chi table, fk and the twiddles.  The element
   // indices of the first and last FFT stages are j + c with c the same for
@@ -194,18 +204,20 @@ __global__ void __launch_bounds__(512)
   // prex[j] prex[c]: one per-thread load, the rest uniform (scalar) loads.
   const C prej = prex[j];
   auto nostore = [&](int, C) {};
-  const int nthr = nv * T;
   const int nv2 = nv * nv;
   const long oy = 2 * y0 - P.ny + 1, oz = 2 * m + 1;  // odd lattice coordinates (|oy| of both rows)
+  // With several slots both rows are run by every slot (a spare row
+  // transforms zeros and stores nothing), so that all threads meet the same
+  // barriers.
   // at most one (kx, -kx) pair per thread and at most 4 sites: fetch this
   // thread's chi row (shared by the two rows) and the fk pair of each row
   // before its forward transform
   if constexpr (SHELL && NVMAX == 4) {
     if (L / 2 <= nthr) {
       Re chr[4][4];
-      const bool mine = tid < L / 2;
+      const bool mine = live && tl < L / 2;
       if (mine) {
-        const long ox = 2 * tid - L + 1;
+        const long ox = 2 * tl - L + 1;
         const long sidx = (ox * ox + oy * oy + oz * oz - 3) >> 3;
         const Re *rowp = chitab + sidx * nv2;
 #pragma unroll
@@ -214,19 +226,25 @@ __global__ void __launch_bounds__(512)
           for (int w = 0; w < 4; ++w) chr[v][w] = (v < nv && w < nv) ? rowp[v * nv + w] : (Re)0;
       }
 #pragma unroll 1
-      for (int r = 0; r < nrows; ++r) {
+      for (int r = 0; r < 2; ++r) {
         // an opaque zero: the table loads of the row's transforms stay in the
         // loop (hoisted out of it, they held ~100 more VGPRs)
         int zero = 0;
         asm volatile("" : "+s"(zero));
         const C *tw = twL + zero, *px = prex + zero;
-        const size_t rowoff = ((size_t)m * P.ny + (r ? y1 : y0)) * L;
+        // a spare row: the workgroup stops before it (one slot), or the slot
+        // runs it on zeros (several slots: per-element guards)
+        const bool rowok = r ? pair : live;
+        if (!kSlots && !rowok) break;
+        const size_t rowoff = rowok ? ((size_t)m * P.ny + (r ? y1 : y0)) * L : 0;
         C *row = spec + (size_t)v * rowstride + rowoff;
         auto load = [&](int i) -> C {
+          if (kSlots && !rowok) return C{0, 0};
           const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
           return cmul(a, cmul(prej, px[i - j]));
         };
         auto store = [&](int i, C vv) {
+          if (kSlots && !rowok) return;
           const C o = cmulc(vv, cmul(prej, px[i - j]));
           __builtin_nontemporal_store(o.x, &row[i].x);
           __builtin_nontemporal_store(o.y, &row[i].y);
@@ -234,19 +252,19 @@ __global__ void __launch_bounds__(512)
         // the difference-form step (mode 3): the long-range term cancels, fk = 0
         const C *fkr = (P.mode == 3) ? nullptr : fkraw + rowoff;
         C fkp[2] = {C{0, 0}, C{0, 0}};
-        if (mine && fkr) {
-          fkp[0] = fkr[tid];
-          fkp[1] = fkr[L - 1 - tid];
+        if (mine && rowok && fkr) {
+          fkp[0] = fkr[tl];
+          fkp[1] = fkr[L - 1 - tl];
         }
         if (r) __syncthreads();  // the first row's inverse stage has read its LDS
         fft_column<L, -1, true, false>(mylds, j, tw, load, nostore);
-        if (mine) {
-          const int off[2] = {lpad(tid), lpad(L - 1 - tid)};
+        if (mine && rowok) {
+          const int off[2] = {lpad(tl), lpad(L - 1 - tl)};
           switch (nv) {
-            case 1: oz_points_reg<1>(lds, CS, off, fkp, P, chr); break;
-            case 2: oz_points_reg<2>(lds, CS, off, fkp, P, chr); break;
-            case 3: oz_points_reg<3>(lds, CS, off, fkp, P, chr); break;
-            default: oz_points_reg<4>(lds, CS, off, fkp, P, chr); break;
+            case 1: oz_points_reg<1>(slds, CS, off, fkp, P, chr); break;
+            case 2: oz_points_reg<2>(slds, CS, off, fkp, P, chr); break;
+            case 3: oz_points_reg<3>(slds, CS, off, fkp, P, chr); break;
+            default: oz_points_reg<4>(slds, CS, off, fkp, P, chr); break;
           }
         }
         __syncthreads();
@@ -256,18 +274,22 @@ __global__ void __launch_bounds__(512)
     }
   }
 #pragma unroll 1
-  for (int r = 0; r < nrows; ++r) {
+  for (int r = 0; r < 2; ++r) {
     int zero = 0;  // an opaque zero, as above
     asm volatile("" : "+s"(zero));
     const C *tw = twL + zero, *px = prex + zero;
+    const bool rowok = r ? pair : live;
+    if (!kSlots && !rowok) break;  // as above
     const int y = r ? y1 : y0;
-    const size_t rowoff = ((size_t)m * P.ny + y) * L;
+    const size_t rowoff = rowok ? ((size_t)m * P.ny + y) * L : 0;
     C *row = spec + (size_t)v * rowstride + rowoff;
     auto load = [&](int i) -> C {
+      if (kSlots && !rowok) return C{0, 0};
       const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
       return cmul(a, cmul(prej, px[i - j]));
     };
     auto store = [&](int i, C vv) {
+      if (kSlots && !rowok) return;
       const C o = cmulc(vv, cmul(prej, px[i - j]));
       __builtin_nontemporal_store(o.x, &row[i].x);
       __builtin_nontemporal_store(o.y, &row[i].y);
@@ -277,18 +299,18 @@ __global__ void __launch_bounds__(512)
     fft_column<L, -1, true, false>(mylds, j, tw, load, nostore);
 
     // OZ at every k of the row, the mirror points (kx, -kx) together
-    for (int jx = tid; jx < L / 2; jx += nthr) {
+    for (int jx = tl; rowok && jx < L / 2; jx += nthr) {
       if constexpr (SHELL) {
         const long ox = 2 * jx - L + 1;
         const long sidx = (ox * ox + oy * oy + oz * oz - 3) >> 3;  // |k|^2 = (pi/L)^2 (8 sidx + 3)
-        oz_row<NVMAX>(lds, CS, jx, L, fkr, P, ChiTable<Re>{chitab + sidx * nv2});
+        oz_row<NVMAX>(slds, CS, jx, L, fkr, P, ChiTable<Re>{chitab + sidx * nv2});
       } else {
         const double ky = P.kfac[1] * ((double)y - (double)P.ny / 2.0 + 0.5);
         const double kz = P.kfac[2] * ((double)(m + P.nz / 2) - (double)P.nz / 2.0 + 0.5);
         const double kx = P.kfac[0] * ((double)jx - (double)L / 2.0 + 0.5);
         const double kk = sqrt(kx * kx + ky * ky + kz * kz);
         const int ii = spline_interval(knots, P.np, P.x0, P.idk, kk);
-        oz_row<NVMAX>(lds, CS, jx, L, fkr, P,
+        oz_row<NVMAX>(slds, CS, jx, L, fkr, P,
                       ChiSpline{coef + (size_t)ii * 4 * nv2, kk - knots[ii], nv2});
       }
     }
@@ -332,10 +354,14 @@ static void launch_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const
                        const cx<Re> *prex, const double *knots, const double *coef,
                        const Re *chitab, hipStream_t st) {
   dim3 grid(P.ny, P.nz / 2, P.grp ? P.ngrp : 1);
-  // the OZ pass takes the rows in mirror pairs (y, ny-1-y)
-  const dim3 pgrid((P.ny + 1) / 2, P.nz / 2, P.grp ? P.ngrp : 1);
   const int threads = P.natv * FftShape<L>::T;
   const size_t lds = (size_t)P.natv * FftShape<L>::CS * sizeof(cx<Re>);
+  // the OZ pass takes the rows in mirror pairs (y, ny-1-y), and short rows
+  // several pairs to a workgroup (row slots), so that it fills a wave
+  const int slots = (FftShape<L>::T >= 64 || threads >= 64) ? 1 : 64 / threads;
+  const dim3 pgrid(((P.ny + 1) / 2 + slots - 1) / slots, P.nz / 2, P.grp ? P.ngrp : 1);
+  const int pthreads = slots * threads;
+  const size_t plds = (size_t)slots * lds;
   if constexpr (sizeof(Re) == sizeof(double)) {
     if (P.mode == 1 || P.mode == 2) {
       if (P.mode == 1)
@@ -348,11 +374,11 @@ static void launch_xoz(const Params &P, cx<Re> *spec, const cx<Re> *fkraw, const
     }
   }
   auto go = [&](auto kern) {
-    if (lds > 65536)
+    if (plds > 65536)
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kern, pgrid, dim3(threads), lds, st, spec, fkraw, P, twL, prex, knots, coef,
-                       chitab);
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds);
+    hipLaunchKernelGGL(kern, pgrid, dim3(pthreads), plds, st, spec, fkraw, P, twL, prex, knots,
+                       coef, chitab);
   };
   if (P.natv <= 4)
     P.shell ? go(k_xpass_oz<L, 4, true, Re>) : go(k_xpass_oz<L, 4, false, Re>);
