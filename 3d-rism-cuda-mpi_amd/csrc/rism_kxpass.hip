@@ -150,9 +150,10 @@ __device__ __forceinline__ void oz_points_reg(cx<Re> *lds, int CS, const int (&o
   }
 }
 
-// One workgroup per m and pair of mirror rows (y, ny-1-y), all sites: natv
-// columns of nx complex in LDS, natv * T threads, the two rows one after the
-// other.  The rows of a pair have the same |ky| and so the same chi rows
+// One workgroup per m and pair of mirror rows (y, ny-1-y) -- several pairs
+// side by side for short rows, see the row slots below -- all sites: natv
+// columns of nx complex in LDS per pair, natv * T threads per pair, the two
+// rows one after the other.  The rows of a pair have the same |ky| and so the same chi rows
 // (the shell index and |k| depend on ky^2 only): the register path loads
 // them once for both rows.  SHELL selects the chi source at compile time (the
 // cubic-box shell table or the spline at |k|) and NVMAX bounds the site
