@@ -1744,9 +1744,18 @@ int rism_set_precision(rism_ctx *c, int prec) {
 
 int rism_get_precision(rism_ctx *c) { return c ? c->prec : -1; }
 
+// RISM_SYNC_FREE=0 keeps rism_solve in the step-by-step loop (the rms read
+// back after every step, as iterate.cu:48-72 does): the loop structure of
+// the reference, for comparisons; any other value or none allows the
+// sync-free path.
+static bool sync_free_allowed() {
+  const char *e = std::getenv("RISM_SYNC_FREE");
+  return !(e && e[0] == '0');
+}
+
 int rism_solve(rism_ctx *c, int maxstep, double tol, int *status, int *steps, double *rms_out) {
   if (!c) return rism::set_error(RISM_ERR_ARG, "null context");
-  if (async_capable(c) && !c->ktime && c->prec == 0 && maxstep > 0) {
+  if (async_capable(c) && !c->ktime && c->prec == 0 && maxstep > 0 && sync_free_allowed()) {
     // same steps without a host round trip per step
     AsyncRun a;
     int rc = async_begin(a, c, maxstep, tol);

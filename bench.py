@@ -491,14 +491,16 @@ def phase_batch(args, budget, R, nsol=256, prec=0, stepwise=True):
         # the same solves one after another with a host round trip per step
         # (the reference's loop structure: rms read back before every update)
         gs = make()
-        for g in gs:
-            g.kernel_timing(True)  # forces the step-by-step loop
-        sync()
-        t0 = time.perf_counter()
-        for g in gs:
-            g.run()
-        sync()
-        t_step = time.perf_counter() - t0
+        os.environ["RISM_SYNC_FREE"] = "0"  # rism_solve's step-by-step loop
+        try:
+            sync()
+            t0 = time.perf_counter()
+            for g in gs:
+                g.run()
+            sync()
+            t_step = time.perf_counter() - t0
+        finally:
+            del os.environ["RISM_SYNC_FREE"]
         for g in gs:
             g.close()
         res["sequential_stepwise_s"] = t_step

@@ -43,6 +43,29 @@ def _stepwise(rism, solv, grid, box, su, closure, maxstep, tol):
     return th, tm, t
 
 
+def test_sync_free_switch(rism, gpu, golden, monkeypatch):
+    """RISM_SYNC_FREE=0 keeps rism_solve in the step-by-step loop (no kernel
+    timing involved); its results equal the sync-free path's bit for bit."""
+    solv = _solvent(rism, golden)
+    grid, box, su = _case(1)
+    out = []
+    for flag in ("0", None):
+        if flag is None:
+            monkeypatch.delenv("RISM_SYNC_FREE", raising=False)
+        else:
+            monkeypatch.setenv("RISM_SYNC_FREE", flag)
+        g = rism.RISM3D(grid, box, solv, su)
+        th, tm = g.run(maxstep=1000, tol=1e-6)
+        out.append((th, tm, g.fields(rism.FIELD_T)))
+        g.close()
+    (th0, tm0, t0), (th1, tm1, t1) = out
+    assert tm0["status"] == tm1["status"] == rism.STATUS_CONVERGED
+    assert tm0["steps"] == tm1["steps"]
+    for k in ("SFE_SC", "SFE_GF", "PMV"):
+        assert th0[k] == th1[k], k
+    assert np.array_equal(t0, t1)
+
+
 def test_batch_equals_stepwise(rism, gpu, golden):
     solv = _solvent(rism, golden)
     cases = [_case(i) for i in range(6)]
