@@ -491,6 +491,7 @@ def phase_batch(args, budget, R, nsol=256, prec=0, stepwise=True):
         # the same solves one after another with a host round trip per step
         # (the reference's loop structure: rms read back before every update)
         gs = make()
+        old = os.environ.get("RISM_SYNC_FREE")
         os.environ["RISM_SYNC_FREE"] = "0"  # rism_solve's step-by-step loop
         try:
             sync()
@@ -500,7 +501,10 @@ def phase_batch(args, budget, R, nsol=256, prec=0, stepwise=True):
             sync()
             t_step = time.perf_counter() - t0
         finally:
-            del os.environ["RISM_SYNC_FREE"]
+            if old is None:
+                del os.environ["RISM_SYNC_FREE"]
+            else:
+                os.environ["RISM_SYNC_FREE"] = old
         for g in gs:
             g.close()
         res["sequential_stepwise_s"] = t_step
