@@ -28,6 +28,15 @@ enum ZKind {
   ZK_PLAIN = 3,   // plain transform of t (forward) or to h (inverse)
 };
 
+// x when A, else y (two objects of different types)
+template <bool A, class X, class Y>
+__device__ __forceinline__ auto &pick_ref(X &x, Y &y) {
+  if constexpr (A)
+    return x;
+  else
+    return y;
+}
+
 // WPE: waves per SIMD the kernel is compiled for (register budget)
 template <int M, int B, int TZ, int MAXR, int KIND, class Re, int WPE = (MAXR == 4 ? 8 : 1)>
 __global__ void __launch_bounds__(B *TZ)
@@ -71,8 +80,8 @@ __global__ void __launch_bounds__(B *TZ)
   // t and u are read once per step: non-temporal loads keep them out of the
   // caches that fr (shared by the sites) and the twiddles use.  (Issuing the
   // loads of several elements before their closures -- the staged protocol
-  // of rism_fft.h -- spills in these 64-VGPR tiles and measured 15-40 %
-  // slower here.)
+  // of rism_fft.h -- spills in the 64-VGPR tiles and measured 15-40 % slower
+  // there; the tiles without a register limit use it, below.)
   auto cval = [&](int z) -> Re {
     const size_t i = base + (size_t)z * plane;
     const Re tt = __builtin_nontemporal_load(&tv[i]);
@@ -92,10 +101,50 @@ __global__ void __launch_bounds__(B *TZ)
   // elements p = j + c with c the same for every thread: zpre[p] = zpre[j]
   // zpre[c], one per-thread load and uniform (scalar) loads
   const C zprej = zpre[j];
-  auto load = [&](int p) -> C {
+  auto plain_load = [&](int p) -> C {
     if (!ok) return C{0, 0};
     return cmul(C{cval(2 * p), cval(2 * p + 1)}, cmul(zprej, zpre[p - j]));
   };
+  // Solver steps in tiles compiled without a register limit: the t, u and
+  // fr values of four elements are fetched together before their closures,
+  // so that their loads are in flight at once (512^3, 4 sites: FP64 wide
+  // radix-8 tile 3.60 -> 3.33 ms, FP32 1.72 -> 1.44 ms)
+  constexpr bool STG = KIND == ZK_STEP && WPE == 1;
+  struct StagedLoad {
+    using staged = void;
+    enum { group = 4 };
+    struct raw_t {
+      Re t0, t1, u0, u1, f0, f1;
+    };
+    const Re *tv, *uv, *fr;
+    const C *zpre;
+    C zprej;
+    size_t base, plane;
+    Re q, dv;
+    int j, clos;
+    bool ok;
+    __device__ __forceinline__ raw_t fetch(int p) const {
+      raw_t r{};
+      if (!ok) return r;
+      const size_t i0 = base + (size_t)(2 * p) * plane, i1 = i0 + plane;
+      r.t0 = __builtin_nontemporal_load(&tv[i0]);
+      r.t1 = __builtin_nontemporal_load(&tv[i1]);
+      r.u0 = __builtin_nontemporal_load(&uv[i0]);
+      r.u1 = __builtin_nontemporal_load(&uv[i1]);
+      r.f0 = fr[i0];
+      r.f1 = fr[i1];
+      return r;
+    }
+    __device__ __forceinline__ Re cv(Re tt, Re uu, Re ff) const {
+      const Re g = closure_g(clos, tt, uu);
+      return (g - (Re)1 - tt + q * ff) * dv;  // trm1mt, times the forward dv
+    }
+    __device__ __forceinline__ C make(const raw_t &r, int p) const {
+      if (!ok) return C{0, 0};
+      return cmul(C{cv(r.t0, r.u0, r.f0), cv(r.t1, r.u1, r.f1)}, cmul(zprej, zpre[p - j]));
+    }
+  } staged_load{tv, uv, fr, zpre, zprej, base, plane, q, dv, j, clos, ok};
+  auto &load = pick_ref<STG>(staged_load, plain_load);
   C *out = spec + ((size_t)v * M * P.ny + y) * P.nx + x;
   const size_t mstride = (size_t)P.ny * P.nx;
   if constexpr (mirror_last_stage<M, TZ, MAXR>()) {
