@@ -240,9 +240,10 @@ __global__ void __launch_bounds__(512)
         const size_t rowoff = rowok ? ((size_t)m * P.ny + (r ? y1 : y0)) * L : 0;
         C *row = spec + (size_t)v * rowstride + rowoff;
         auto load = [&](int i) -> C {
-          if (kSlots && !rowok) return C{0, 0};
+          // rowoff is 0 for a spare row: the load stays in bounds, unguarded
           const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
-          return cmul(a, cmul(prej, px[i - j]));
+          const C w = cmul(a, cmul(prej, px[i - j]));
+          return (kSlots && !rowok) ? C{0, 0} : w;
         };
         auto store = [&](int i, C vv) {
           if (kSlots && !rowok) return;
@@ -285,9 +286,10 @@ __global__ void __launch_bounds__(512)
     const size_t rowoff = rowok ? ((size_t)m * P.ny + y) * L : 0;
     C *row = spec + (size_t)v * rowstride + rowoff;
     auto load = [&](int i) -> C {
-      if (kSlots && !rowok) return C{0, 0};
+      // rowoff is 0 for a spare row: the load stays in bounds, unguarded
       const C a{__builtin_nontemporal_load(&row[i].x), __builtin_nontemporal_load(&row[i].y)};
-      return cmul(a, cmul(prej, px[i - j]));
+      const C w = cmul(a, cmul(prej, px[i - j]));
+      return (kSlots && !rowok) ? C{0, 0} : w;
     };
     auto store = [&](int i, C vv) {
       if (kSlots && !rowok) return;

@@ -31,11 +31,14 @@ __global__ void __launch_bounds__(B *FftShape<L>::T)
   // pre[j + c] = pre[j] pre[c] (an exponential; c is the same for every
   // thread in the first and last stages): one per-thread load
   const C prej = pre[j];
+  // col points at column 0 when x is past nx, so every load stays in bounds
+  // and is issued unguarded: a load guarded by `ok` waits for its data inside
+  // the guard, one element after another
   auto load = [&](int i) -> C {
-    if (!ok) return C{0, 0};
     const C *q = col + (size_t)i * nx;
     const C vv{__builtin_nontemporal_load(&q->x), __builtin_nontemporal_load(&q->y)};
-    return (S < 0) ? cmul(vv, cmul(prej, pre[i - j])) : vv;
+    const C w = (S < 0) ? cmul(vv, cmul(prej, pre[i - j])) : vv;
+    return ok ? w : C{0, 0};
   };
   auto store = [&](int i, C vv) {
     if (!ok) return;

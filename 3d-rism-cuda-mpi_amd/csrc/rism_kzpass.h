@@ -101,9 +101,18 @@ __global__ void __launch_bounds__(B *TZ)
   // elements p = j + c with c the same for every thread: zpre[p] = zpre[j]
   // zpre[c], one per-thread load and uniform (scalar) loads
   const C zprej = zpre[j];
+  // base points at column 0 when x is past nx: the loads stay in bounds and
+  // are issued unguarded (a guarded load waits for its data inside the guard,
+  // one element after another), except in the register-capped tiles, where
+  // the unguarded loads spill
   auto plain_load = [&](int p) -> C {
-    if (!ok) return C{0, 0};
-    return cmul(C{cval(2 * p), cval(2 * p + 1)}, cmul(zprej, zpre[p - j]));
+    if constexpr (WPE > 1) {
+      if (!ok) return C{0, 0};
+      return cmul(C{cval(2 * p), cval(2 * p + 1)}, cmul(zprej, zpre[p - j]));
+    } else {
+      const C w = cmul(C{cval(2 * p), cval(2 * p + 1)}, cmul(zprej, zpre[p - j]));
+      return ok ? w : C{0, 0};
+    }
   };
   // Solver steps in tiles compiled without a register limit: the t, u and
   // fr values of four elements are fetched together before their closures,
@@ -124,8 +133,7 @@ __global__ void __launch_bounds__(B *TZ)
     int j, clos;
     bool ok;
     __device__ __forceinline__ raw_t fetch(int p) const {
-      raw_t r{};
-      if (!ok) return r;
+      raw_t r;
       const size_t i0 = base + (size_t)(2 * p) * plane, i1 = i0 + plane;
       r.t0 = __builtin_nontemporal_load(&tv[i0]);
       r.t1 = __builtin_nontemporal_load(&tv[i1]);
@@ -270,10 +278,8 @@ __global__ void __launch_bounds__(B *TZ)
   // zpost[m] = zpost[j] zpre[m - j] and zpost[M-1-m] = -conj(zpost[m])
   // (e^{-i pi (2m+1)/nz}, nz = 2M)
   const C zpostj = zpost[j];
-  for (int m = j; 2 * m <= M - 1; m += T) {
+  auto untangle = [&](int m, C a, C b) {
     const int mm = M - 1 - m;
-    const C a = ok ? in[(size_t)m * plane] : C{0, 0};
-    const C b = ok ? in[(size_t)mm * plane] : C{0, 0};
     const C wm = cmul(zpostj, zpre[m - j]);
     const C wmm{-wm.x, wm.y};
     const C Em = (Re)0.5 * (a + conj(b));
@@ -282,6 +288,34 @@ __global__ void __launch_bounds__(B *TZ)
     const C Omm = cmulc((Re)0.5 * (b - conj(a)), wmm);
     col[lpad(m)] = Em + mul_si<1>(Om);
     if (mm != m) col[lpad(mm)] = Emm + mul_si<1>(Omm);
+  };
+  if constexpr (sizeof(Re) == 8) {
+    // all pairs of this thread are loaded before any is untangled, so their
+    // loads are in flight together (in points at column 0 when x is past
+    // nx: the loads stay in bounds and need no guard); the register-capped
+    // FP32 tiles would spill for it and load pair by pair
+    constexpr int NPAIR = (M + 1) / 2;  // m with 2m <= M-1
+    constexpr int PPT = (NPAIR + T - 1) / T;
+    C av[PPT], bv[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int m = j + k * T;
+      if (NPAIR % T == 0 || m < NPAIR) {
+        av[k] = in[(size_t)m * plane];
+        bv[k] = in[(size_t)(M - 1 - m) * plane];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const int m = j + k * T;
+      if (NPAIR % T == 0 || m < NPAIR)
+        untangle(m, ok ? av[k] : C{0, 0}, ok ? bv[k] : C{0, 0});
+    }
+  } else {
+    for (int m = j; 2 * m <= M - 1; m += T) {
+      const int mm = M - 1 - m;
+      untangle(m, ok ? in[(size_t)m * plane] : C{0, 0}, ok ? in[(size_t)mm * plane] : C{0, 0});
+    }
   }
   __syncthreads();
 
