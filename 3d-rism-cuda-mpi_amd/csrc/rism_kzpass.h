@@ -103,10 +103,11 @@ __global__ void __launch_bounds__(B *TZ)
   const C zprej = zpre[j];
   // base points at column 0 when x is past nx: the loads stay in bounds and
   // are issued unguarded (a guarded load waits for its data inside the guard,
-  // one element after another), except in the register-capped tiles, where
-  // the unguarded loads spill
+  // one element after another), except in FP32 and the register-capped
+  // tiles: there the unguarded loads spill or push the FP32 tile past the 64
+  // VGPRs that let two workgroups share a CU
   auto plain_load = [&](int p) -> C {
-    if constexpr (WPE > 1) {
+    if constexpr (WPE > 1 || sizeof(Re) == 4) {
       if (!ok) return C{0, 0};
       return cmul(C{cval(2 * p), cval(2 * p + 1)}, cmul(zprej, zpre[p - j]));
     } else {
